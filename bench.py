@@ -1,0 +1,207 @@
+"""Benchmark: batched slog_pfaffian throughput on an uneven batch (one GPU).
+
+Workload (the headline metric "batched Pf/s (N 16-256)"): 16384 float64
+skew-symmetric matrices, N drawn uniformly from the even numbers 16..256,
+entries N(0,1) skew-symmetrised A = (B - B^T)/2, seeded, generated on the GPU
+and resident in HBM before timing.  One step = one pfh_slog_pfaffian_ragged
+call over the whole batch (device-side size bucketing + one persistent launch
+per bucket).  Prints one JSON line.
+
+Also reported: the small batched product path (1024 x 32x32 f64), the
+dominant kernel's roofline fraction (HIP events on the launch stream), the
+accuracy of a sample against the FP64 CPU oracle, and the CPU oracle's own
+throughput on that sample (cpu_baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK = 78.6e12      # MI355X FP64 (vector = matrix) FLOP/s
+HBM_PEAK = 8.0e12        # MI355X HBM3E bytes/s
+
+
+def gen_ragged(torch, count, lo, hi, seed, dtype, dev):
+    """Sizes uniform over even lo..hi; data grouped by size on the device."""
+    rng = np.random.default_rng(seed)
+    sizes = (2 * rng.integers(lo // 2, hi // 2 + 1, size=count)).astype(np.int32)
+    offsets = np.zeros(count, np.int64)
+    order = np.argsort(sizes, kind="stable")
+    pos = 0
+    for i in order:
+        offsets[i] = pos
+        pos += int(sizes[i]) ** 2
+    data = torch.empty(pos, dtype=dtype, device=dev)
+    g = torch.Generator(device=dev)
+    for n in np.unique(sizes):
+        idx = order[sizes[order] == n]
+        base = int(offsets[idx[0]])
+        k = len(idx)
+        g.manual_seed(seed * 1000003 + int(n))
+        if dtype.is_complex:
+            rdt = torch.float64 if dtype == torch.complex128 else torch.float32
+            B = torch.complex(torch.randn(k, n, n, generator=g, device=dev, dtype=rdt),
+                              torch.randn(k, n, n, generator=g, device=dev, dtype=rdt)) / math.sqrt(2)
+        else:
+            B = torch.randn(k, n, n, generator=g, device=dev, dtype=dtype)
+        data[base:base + k * n * n] = ((B - B.transpose(1, 2)) / 2).reshape(-1)
+    return data, offsets, sizes
+
+
+class Ragged:
+    """Direct C-ABI caller with preallocated workspace and outputs."""
+
+    def __init__(self, torch, lib, data, offsets, sizes, dt):
+        self.torch, self.lib = torch, lib
+        dev = data.device
+        self.data = data
+        self.off = torch.from_numpy(offsets).to(dev)
+        self.siz = torch.from_numpy(sizes).to(dev)
+        self.count = len(sizes)
+        self.max_n = int(sizes.max())
+        self.dt = dt
+        self.log = torch.empty(self.count, dtype=torch.float64, device=dev)
+        self.ph = torch.empty(self.count, dtype=data.dtype, device=dev)
+        import ctypes
+        need = ctypes.c_size_t(0)
+        lib.pfh_slog_pfaffian_ragged_workspace_size(dt, self.count, self.max_n, need)
+        self.ws = torch.empty(max(int(need.value), 1), dtype=torch.uint8, device=dev)
+        self.stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def __call__(self):
+        st = self.lib.pfh_slog_pfaffian_ragged(
+            self.dt, self.data.data_ptr(), self.off.data_ptr(), self.siz.data_ptr(), self.count,
+            self.max_n, self.log.data_ptr(), self.ph.data_ptr(), self.ws.data_ptr(), self.ws.numel(),
+            self.stream)
+        if st != 0:
+            raise RuntimeError(f"pfh_slog_pfaffian_ragged status {st}")
+
+
+def time_events(torch, fn, reps):
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    e.synchronize()
+    return s.elapsed_time(e) / 1e3 / reps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--count", type=int, default=16384)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-sample", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the batch would shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import pfcuda_amd
+    from pfcuda_amd import _lib
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    lib = _lib.lib
+    t0 = time.time()
+    data, offsets, sizes = gen_ragged(torch, args.count, 16, 256, args.seed, torch.float64, dev)
+    torch.cuda.synchronize()
+    print(f"# generated {args.count} matrices ({data.numel() * 8 / 1e9:.2f} GB) in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+
+    run = Ragged(torch, lib, data, offsets, sizes, _lib.PFH_F64)
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    ms = time_events(torch, run, args.steps) * 1e3
+    value = args.count / (ms / 1e3)
+    print(f"# ragged: {ms:.3f} ms/step", file=sys.stderr, flush=True)
+
+    flops = float(np.sum(sizes.astype(np.float64) ** 3) / 3.0)
+    tri_bytes = float(np.sum(sizes.astype(np.float64) * (sizes - 1) / 2.0) * 8)
+
+    # dominant kernel: the largest register bucket (192 < N <= 256)
+    big = np.nonzero(sizes > 192)[0]
+    sub = Ragged(torch, lib, data, offsets[big], sizes[big], _lib.PFH_F64)
+    sub()
+    torch.cuda.synchronize()
+    t_big = time_events(torch, sub, 5)
+    big_flops = float(np.sum(sizes[big].astype(np.float64) ** 3) / 3.0)
+    achieved = big_flops / t_big
+
+    # small batched product path: 1024 x 32x32 f64
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed + 1)
+    B = torch.randn(1024, 32, 32, generator=g, device=dev, dtype=torch.float64)
+    A1 = ((B - B.transpose(1, 2)) / 2).contiguous()
+    out1 = torch.empty(1024, dtype=torch.float64, device=dev)
+    strm = torch.cuda.current_stream(dev).cuda_stream
+
+    def c1():
+        lib.pfh_pfaffian_f64(A1.data_ptr(), 1024, 32, out1.data_ptr(), strm)
+    c1()
+    torch.cuda.synchronize()
+    t_c1 = time_events(torch, c1, 20)
+
+    # accuracy of a sample against the FP64 CPU oracle, and its CPU time
+    res = {}
+    cpu_baseline = None
+    if not args.no_cpu:
+        import oracle
+        k = min(args.cpu_sample, args.count)
+        idx = np.arange(k)
+        host = data.cpu().numpy()
+        sub_sizes = sizes[idx]
+        threads = max(1, min(16, os.cpu_count() or 1))
+        tc = time.time()
+        wl, ws = oracle.slog_ragged(host, offsets[idx], sub_sizes, threads=threads)
+        tcpu = time.time() - tc
+        gl = run.log.cpu().numpy()[idx]
+        gs = run.ph.cpu().numpy()[idx]
+        res = {"max_abs_dlog": float(np.max(np.abs(gl - wl))),
+               "sign_mismatches": int(np.sum(gs != ws)), "checked": int(k)}
+        cpu_baseline = {"value": k / tcpu, "unit": "Pf/s", "cores": threads, "kind": "port",
+                        "sample": f"first {k} of the {args.count} matrices (N 16-256 f64), "
+                                  f"{threads} threads, one matrix per task, {tcpu:.1f} s"}
+
+    line = {
+        "metric": "batched Pf/s (N 16-256); slog_pf N=8192 ms; mixed vs FP64, |dlog|<=1e-9",
+        "value": value, "unit": "Pf/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded N(0,1), A=(B-B^T)/2, generated on GPU)",
+        "config": {"workload": "ragged slog_pfaffian: 16384 f64 skew matrices, even N uniform in [16,256]",
+                   "count": args.count, "seed": args.seed, "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK, "traffic": None,
+                     "kernel": "pf_reg_kernel<double,256,4,64,slog> (bucket 192<N<=256)",
+                     "launch_ms": t_big * 1e3, "algorithmic_flop": big_flops},
+        "whole_batch": {"flop": flops, "tflops": flops / (ms / 1e3) / 1e12,
+                        "frac_fp64_peak": flops / (ms / 1e3) / FP64_PEAK, "triangle_bytes": tri_bytes},
+        "small_batched_c1": {"workload": "pfaffian() 1024 x 32x32 f64", "ms": t_c1 * 1e3,
+                             "pf_per_s": 1024 / t_c1,
+                             "hbm_frac": (1024 * 32 * 31 / 2 * 8) / t_c1 / HBM_PEAK},
+        "accuracy_vs_oracle": res,
+        "cpu_baseline": cpu_baseline,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

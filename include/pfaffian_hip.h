@@ -1,0 +1,101 @@
+/* C ABI of the MI355X (gfx950) Pfaffian library, libpfaffian_hip.so.
+ *
+ * Every entry point takes device pointers, sizes and a hipStream_t, enqueues
+ * its work on that stream, never synchronises the host, never throws, and
+ * returns 0 on success, a negative PFH_E* code for a rejected argument, or a
+ * positive hipError_t value.  Matrices are dense row-major n x n slabs packed
+ * back to back (the logical matrix A[i][j] at i*n + j).  Complex data are
+ * interleaved (re, im) pairs of float (c64) or double (c128).
+ *
+ * Zero pivots are detected exactly (pivot == 0) and give Pf = 0, log|Pf| =
+ * -inf and sign/phase = 0; odd n gives the same.
+ */
+#ifndef PFAFFIAN_HIP_H
+#define PFAFFIAN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFH_OK 0
+#define PFH_EINVAL (-1)      /* bad size, count or pointer */
+#define PFH_EWORKSPACE (-2)  /* workspace too small */
+#define PFH_ETOOBIG (-3)     /* n beyond what the entry point supports */
+
+/* dtype codes */
+#define PFH_F32 0
+#define PFH_F64 1
+#define PFH_C64 2
+#define PFH_C128 3
+
+/* slog methods */
+#define PFH_METHOD_AUTO 0
+#define PFH_METHOD_UNBLOCKED 1   /* plain all-FP64 path, one rank-2 update per pair step */
+#define PFH_METHOD_BLOCKED 2     /* blocked, FP64 trailing update */
+#define PFH_METHOD_MIXED 3       /* blocked, split-precision trailing update */
+
+/* Short text for a status code. */
+const char* pfh_status_string(int status);
+
+/* Largest n that pfh_pfaffian_* accepts (64). */
+int pfh_pfaffian_max_n(void);
+
+/* Pf of `batch` matrices of size n (product form, the input dtype).
+ * Replaces the XLA FFI handlers pfaffian_{f32,f64,c64,c128}
+ * (reference bindings/jax_bindings.cu:82-112 and :152-182), which loop over
+ * the batch launching one 32-thread block per matrix
+ * (src/pfaffian.cu:42-48, src/pfaffian_sm.cu:101-105).  n <= 64. */
+int pfh_pfaffian_f32(const float* A, int64_t batch, int32_t n, float* out, hipStream_t stream);
+int pfh_pfaffian_f64(const double* A, int64_t batch, int32_t n, double* out, hipStream_t stream);
+int pfh_pfaffian_c64(const void* A, int64_t batch, int32_t n, void* out, hipStream_t stream);
+int pfh_pfaffian_c128(const void* A, int64_t batch, int32_t n, void* out, hipStream_t stream);
+
+/* Workspace bytes needed by pfh_slog_pfaffian_* for (dtype, batch, n, method). */
+int pfh_slog_pfaffian_workspace_size(int dtype, int64_t batch, int32_t n, int method, size_t* bytes);
+
+/* log|Pf| (double) and sign or unit phase (input dtype) of `batch` matrices
+ * of size n.  Replaces the XLA FFI handlers slog_pfaffian_{f32,f64,c64,c128}
+ * (reference bindings/jax_bindings.cu:19-80 and :114-149) and the
+ * host-driven slog_pfaffian_lg (src/slog_pfaffian_lg.cu:151-211).  The log
+ * output is double for every dtype.  The input is never modified.
+ * `workspace` may be NULL, in which case the library uses a cached
+ * per-device buffer grown with hipMalloc (not capture-safe). */
+int pfh_slog_pfaffian_f32(const float* A, int64_t batch, int32_t n, double* log_out, float* sign_out,
+                          void* workspace, size_t workspace_bytes, hipStream_t stream);
+int pfh_slog_pfaffian_f64(const double* A, int64_t batch, int32_t n, double* log_out, double* sign_out,
+                          void* workspace, size_t workspace_bytes, hipStream_t stream);
+int pfh_slog_pfaffian_c64(const void* A, int64_t batch, int32_t n, double* log_out, void* phase_out,
+                          void* workspace, size_t workspace_bytes, hipStream_t stream);
+int pfh_slog_pfaffian_c128(const void* A, int64_t batch, int32_t n, double* log_out, void* phase_out,
+                           void* workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* Same, with an explicit method (PFH_METHOD_*) for matrices past the
+ * register-resident kernels. No counterpart in the reference: it selects
+ * between the plain and blocked forms of its slog_pfaffian_lg. */
+int pfh_slog_pfaffian_method(int dtype, const void* A, int64_t batch, int32_t n, int method,
+                             double* log_out, void* phase_out,
+                             void* workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* Workspace bytes for a ragged batch of `count` matrices no larger than max_n. */
+int pfh_slog_pfaffian_ragged_workspace_size(int dtype, int64_t count, int32_t max_n, size_t* bytes);
+
+/* log|Pf| and sign/phase of a ragged batch in one pass: matrix i is the
+ * sizes[i] x sizes[i] row-major slab starting at element offsets[i] of
+ * `data`.  offsets and sizes are device arrays; max_n bounds every size.
+ * Matrices are bucketed by size on the device and each bucket runs as one
+ * persistent launch, so the call has no host synchronisation.  No
+ * counterpart in the reference, whose handlers take one dense shape and
+ * loop over it serially (bindings/jax_bindings.cu:52-72). */
+int pfh_slog_pfaffian_ragged(int dtype, const void* data, const int64_t* offsets, const int32_t* sizes,
+                             int64_t count, int32_t max_n, double* log_out, void* phase_out,
+                             void* workspace, size_t workspace_bytes, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFAFFIAN_HIP_H */

@@ -1,0 +1,17 @@
+"""MI355X-native Pfaffian library: HIP kernels for gfx950 behind a C ABI.
+
+Public surface (mirrors the reference package pfcuda, pfcuda/__init__.py:1-5):
+  pfaffian, slog_pfaffian             -- as pfcuda.pfaffian / pfcuda.slog_pfaffian
+  slog_pfaffian_batched               -- slog without the n > 32 guard
+  slog_pfaffian_ragged, pack_ragged   -- uneven batches in one call
+Importing this package loads libpfaffian_hip.so and fails if it is absent.
+"""
+from ._lib import PFH_METHOD_AUTO, PFH_METHOD_BLOCKED, PFH_METHOD_MIXED, PFH_METHOD_UNBLOCKED
+from .api import (PFAFFIAN_MAX_N, pack_ragged, pfaffian, slog_pfaffian, slog_pfaffian_batched,
+                  slog_pfaffian_ragged)
+
+__all__ = [
+    "pfaffian", "slog_pfaffian", "slog_pfaffian_batched", "slog_pfaffian_ragged", "pack_ragged",
+    "PFAFFIAN_MAX_N", "PFH_METHOD_AUTO", "PFH_METHOD_UNBLOCKED", "PFH_METHOD_BLOCKED",
+    "PFH_METHOD_MIXED",
+]

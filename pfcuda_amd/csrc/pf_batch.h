@@ -1,0 +1,37 @@
+// Internal launcher interfaces shared by the kernel translation units and
+// the C-ABI layer.  Not part of the public C-ABI (see include/pfaffian_hip.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pfh {
+
+// One batch of matrices for the workgroup kernels.  Matrix j of the launch
+// is order[j] when `order` is set (a size bucket), else j.  Ragged batches
+// give per-matrix element offsets and sizes; dense batches give dense_n and
+// matrices packed back to back.  Every matrix is row-major n x n.
+struct BatchDesc {
+    const void* data;
+    const int64_t* offsets;   // ragged: element offset of each matrix (nullptr: dense)
+    const int32_t* sizes;     // ragged: n of each matrix (nullptr: dense_n)
+    const int32_t* order;     // optional list of matrix indices to process
+    const int32_t* begin_dev; // optional device-side start of this launch's range in `order`
+    const int32_t* count_dev; // optional device-side length of that range
+    int64_t count;            // number of entries (upper bound when count_dev is set)
+    int32_t dense_n;
+    void* prod_out;           // MODE_PROD: Pf as the input type
+    double* log_out;          // MODE_SLOG: log|Pf|
+    void* phase_out;          // MODE_SLOG: sign / unit phase as the input type
+};
+
+enum { MODE_PROD = 0, MODE_SLOG = 1 };
+enum { DT_F32 = 0, DT_F64 = 1, DT_C64 = 2, DT_C128 = 3 };
+
+// Largest n the register-resident workgroup kernels take for a dtype.
+int reg_max_n(int dtype);
+
+// Launches the workgroup kernel whose capacity covers `maxn` for every
+// matrix in the batch.  Returns 0 or a hipError_t value.
+int launch_reg(int dtype, int mode, const BatchDesc& d, int maxn, hipStream_t stream);
+
+}  // namespace pfh

@@ -1,0 +1,353 @@
+// C-ABI entry points (declared in include/pfaffian_hip.h) and the
+// device-side size bucketing of ragged batches.
+#include <mutex>
+#include <vector>
+
+#include "../../include/pfaffian_hip.h"
+#include "pf_batch.h"
+#include "pf_large.h"
+
+using namespace pfh;
+
+namespace {
+
+size_t dtype_bytes(int dt) {
+    switch (dt) {
+        case PFH_F32: return 4;
+        case PFH_F64: return 8;
+        case PFH_C64: return 8;
+        case PFH_C128: return 16;
+    }
+    return 0;
+}
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Cached per-device workspace for callers that pass none.
+struct WsCache {
+    std::mutex mu;
+    std::vector<void*> ptr;
+    std::vector<size_t> bytes;
+};
+WsCache g_ws;
+
+int cached_workspace(size_t need, void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    std::lock_guard<std::mutex> lk(g_ws.mu);
+    if ((int)g_ws.ptr.size() <= dev) { g_ws.ptr.resize(dev + 1, nullptr); g_ws.bytes.resize(dev + 1, 0); }
+    if (g_ws.bytes[dev] < need) {
+        if (g_ws.ptr[dev]) {
+            e = hipDeviceSynchronize();
+            if (e != hipSuccess) return (int)e;
+            hipFree(g_ws.ptr[dev]);
+            g_ws.ptr[dev] = nullptr;
+            g_ws.bytes[dev] = 0;
+        }
+        e = hipMalloc(&g_ws.ptr[dev], need);
+        if (e != hipSuccess) { g_ws.ptr[dev] = nullptr; return (int)e; }
+        g_ws.bytes[dev] = need;
+    }
+    *out = g_ws.ptr[dev];
+    return 0;
+}
+
+int resolve_ws(void* ws, size_t ws_bytes, size_t need, void** out) {
+    if (need == 0) { *out = ws; return 0; }
+    if (ws) {
+        if (ws_bytes < need) return PFH_EWORKSPACE;
+        *out = ws;
+        return 0;
+    }
+    return cached_workspace(need, out);
+}
+
+// ---------------------------------------------------------------------------
+// ragged bucketing: counting sort of matrix indices by size, largest first
+// ---------------------------------------------------------------------------
+constexpr int kMaxBuckets = 8;
+
+__global__ void rg_zero(int32_t* hist, int nbins) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nbins; i += gridDim.x * blockDim.x) hist[i] = 0;
+}
+
+__global__ void rg_hist(const int32_t* sizes, int64_t count, int32_t max_n, int32_t* hist) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+        int s = sizes[i];
+        s = s < 0 ? 0 : (s > max_n ? max_n : s);
+        atomicAdd(&hist[s], 1);
+    }
+}
+
+// One thread: positions in descending-size order and bucket ranges.
+// Bucket b holds sizes in (edge[b-1], edge[b]] (edge[-1] = -1).
+__global__ void rg_scan(int32_t* hist, int32_t max_n, const int32_t* edges, int nb,
+                        int32_t* cursor, int32_t* bucket_begin, int32_t* bucket_count) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int32_t pos = 0;
+    for (int s = max_n; s >= 0; --s) {
+        cursor[s] = pos;
+        pos += hist[s];
+    }
+    for (int b = 0; b < nb; ++b) {
+        const int hi = edges[b] > max_n ? max_n : edges[b];
+        const int lo = b == 0 ? -1 : edges[b - 1];
+        if (hi <= lo) { bucket_begin[b] = 0; bucket_count[b] = 0; continue; }
+        const int32_t begin = cursor[hi];
+        const int32_t end = (lo >= 0) ? cursor[lo] : pos;
+        bucket_begin[b] = begin;
+        bucket_count[b] = end - begin;
+    }
+}
+
+__global__ void rg_scatter(const int32_t* sizes, int64_t count, int32_t max_n, int32_t* cursor, int32_t* order) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+        int s = sizes[i];
+        s = s < 0 ? 0 : (s > max_n ? max_n : s);
+        const int slot = atomicAdd(&cursor[s], 1);
+        order[slot] = (int32_t)i;
+    }
+}
+
+// Register-kernel bucket edges per dtype (the capacities of pf_reg.hip).
+int bucket_edges(int dt, int32_t* e) {
+    switch (dt) {
+        case PFH_F64: e[0] = 32; e[1] = 64; e[2] = 128; e[3] = 192; e[4] = 256; return 5;
+        case PFH_F32: e[0] = 32; e[1] = 64; e[2] = 128; e[3] = 256; return 4;
+        case PFH_C64: e[0] = 32; e[1] = 64; e[2] = 128; e[3] = 192; return 4;
+        case PFH_C128: e[0] = 32; e[1] = 64; e[2] = 128; return 3;
+    }
+    return 0;
+}
+
+struct RaggedLayout {
+    size_t hist, cursor, order, begin, cnt, edges, slabs, total;
+    int glb_grid;
+    int64_t slab_elems;
+};
+
+int glb_grid() {
+    int dev = 0, v = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    return 2 * v;
+}
+
+RaggedLayout ragged_layout(int dt, int64_t count, int32_t max_n) {
+    RaggedLayout L{};
+    size_t off = 0;
+    L.hist = off; off = align_up(off + sizeof(int32_t) * (max_n + 1));
+    L.cursor = off; off = align_up(off + sizeof(int32_t) * (max_n + 1));
+    L.order = off; off = align_up(off + sizeof(int32_t) * (count > 0 ? count : 1));
+    L.begin = off; off = align_up(off + sizeof(int32_t) * (kMaxBuckets + 1));
+    L.cnt = off; off = align_up(off + sizeof(int32_t) * (kMaxBuckets + 1));
+    L.edges = off; off = align_up(off + sizeof(int32_t) * (kMaxBuckets + 1));
+    L.slabs = off;
+    L.glb_grid = 0;
+    L.slab_elems = 0;
+    if (max_n > reg_max_n(dt)) {
+        L.glb_grid = glb_grid();
+        L.slab_elems = (int64_t)max_n * max_n;
+        off = align_up(off + dtype_bytes(dt) * (size_t)L.slab_elems * L.glb_grid);
+    }
+    L.total = off;
+    return L;
+}
+
+__global__ void rg_set_edges(int32_t* dst, int32_t e0, int32_t e1, int32_t e2, int32_t e3, int32_t e4, int32_t e5) {
+    if (threadIdx.x == 0) { dst[0] = e0; dst[1] = e1; dst[2] = e2; dst[3] = e3; dst[4] = e4; dst[5] = e5; }
+}
+
+template <typename T>
+int ragged_typed(int dt, const void* data, const int64_t* offsets, const int32_t* sizes, int64_t count,
+                 int32_t max_n, double* log_out, void* phase_out, char* ws, const RaggedLayout& L,
+                 hipStream_t s) {
+    int32_t* hist = (int32_t*)(ws + L.hist);
+    int32_t* cursor = (int32_t*)(ws + L.cursor);
+    int32_t* order = (int32_t*)(ws + L.order);
+    int32_t* bbegin = (int32_t*)(ws + L.begin);
+    int32_t* bcount = (int32_t*)(ws + L.cnt);
+    int32_t* edges = (int32_t*)(ws + L.edges);
+
+    int32_t e[kMaxBuckets];
+    int nb = bucket_edges(dt, e);
+    const int rmax = reg_max_n(dt);
+    int ne = nb;
+    if (max_n > rmax) e[ne++] = max_n;   // the global-slab bucket
+    for (int i = ne; i < 6; ++i) e[i] = e[ne - 1];
+    hipLaunchKernelGGL(rg_set_edges, dim3(1), dim3(64), 0, s, edges, e[0], e[1], e[2], e[3], e[4], e[5]);
+    hipLaunchKernelGGL(rg_zero, dim3(1), dim3(256), 0, s, hist, max_n + 1);
+    const int g = (int)((count + 255) / 256 > 1024 ? 1024 : (count + 255) / 256);
+    hipLaunchKernelGGL(rg_hist, dim3(g), dim3(256), 0, s, sizes, count, max_n, hist);
+    hipLaunchKernelGGL(rg_scan, dim3(1), dim3(64), 0, s, hist, max_n, edges, ne, cursor, bbegin, bcount);
+    hipLaunchKernelGGL(rg_scatter, dim3(g), dim3(256), 0, s, sizes, count, max_n, cursor, order);
+    int err = (int)hipGetLastError();
+    if (err) return err;
+
+    // Largest buckets first so their long tails overlap the small ones'
+    // launches as little as possible on the one stream.
+    for (int b = ne - 1; b >= 0; --b) {
+        const int lo = b == 0 ? 0 : e[b - 1];
+        if (lo >= max_n) continue;          // bucket entirely above max_n: empty
+        BatchDesc d{};
+        d.data = data; d.offsets = offsets; d.sizes = sizes;
+        d.order = order; d.begin_dev = bbegin + b; d.count_dev = bcount + b;
+        d.count = count; d.dense_n = 0;
+        d.log_out = log_out; d.phase_out = phase_out;
+        if (b >= nb) {
+            err = launch_glb<T>(d, (T*)(ws + L.slabs), L.slab_elems, L.glb_grid, s);
+        } else {
+            err = launch_reg(dt, MODE_SLOG, d, e[b], s);
+        }
+        if (err) return err;
+    }
+    return 0;
+}
+
+int check_dense(int64_t batch, int32_t n) {
+    if (batch < 0 || n < 0) return PFH_EINVAL;
+    return 0;
+}
+
+template <typename T>
+int pfaffian_dense(int dt, const void* A, int64_t batch, int32_t n, void* out, hipStream_t s) {
+    if (check_dense(batch, n)) return PFH_EINVAL;
+    if (n > 64) return PFH_ETOOBIG;
+    if (batch == 0) return 0;
+    if (!A || !out) return PFH_EINVAL;
+    BatchDesc d{};
+    d.data = A; d.count = batch; d.dense_n = n; d.prod_out = out;
+    return launch_reg(dt, MODE_PROD, d, n < 2 ? 2 : n, s);
+}
+
+size_t slog_ws_bytes(int dt, int64_t batch, int32_t n, int method) {
+    if (n <= reg_max_n(dt) && method == PFH_METHOD_AUTO) return 0;
+    if (n <= reg_max_n(dt) && method != PFH_METHOD_AUTO) { /* forced large path */ }
+    const size_t es = dtype_bytes(dt);
+    return align_up(es * (size_t)n * n) + align_up(sizeof(LargeState<c128>));
+}
+
+template <typename T>
+int slog_dense(int dt, const T* A, int64_t batch, int32_t n, int method, double* log_out, T* phase_out,
+               void* ws_in, size_t ws_bytes, hipStream_t s) {
+    if (check_dense(batch, n)) return PFH_EINVAL;
+    if (method < PFH_METHOD_AUTO || method > PFH_METHOD_MIXED) return PFH_EINVAL;
+    if (batch == 0) return 0;
+    if (!A || !log_out || !phase_out) return PFH_EINVAL;
+    if (method == PFH_METHOD_AUTO && n <= reg_max_n(dt)) {
+        BatchDesc d{};
+        d.data = A; d.count = batch; d.dense_n = n; d.log_out = log_out; d.phase_out = phase_out;
+        return launch_reg(dt, MODE_SLOG, d, n < 2 ? 2 : n, s);
+    }
+    if (n == 0 || (n & 1)) {
+        // odd/empty: one register launch writes (-inf, 0)
+        BatchDesc d{};
+        d.data = A; d.count = batch; d.dense_n = n; d.log_out = log_out; d.phase_out = phase_out;
+        return launch_reg(dt, MODE_SLOG, d, 2, s);
+    }
+    void* ws = nullptr;
+    const size_t need = slog_ws_bytes(dt, batch, n, method);
+    int err = resolve_ws(ws_in, ws_bytes, need, &ws);
+    if (err) return err;
+    if (method == PFH_METHOD_BLOCKED || method == PFH_METHOD_MIXED) return PFH_EINVAL;
+    for (int64_t b = 0; b < batch; ++b) {
+        const T* Ab = A + b * (int64_t)n * n;
+        T* W = (T*)ws;
+        LargeState<T>* st = (LargeState<T>*)((char*)ws + align_up(sizeof(T) * (size_t)n * n));
+        err = large_slog<T>(Ab, n, W, st, log_out + b, phase_out + b, s);
+        if (err) return err;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pfh_status_string(int status) {
+    switch (status) {
+        case PFH_OK: return "success";
+        case PFH_EINVAL: return "invalid argument";
+        case PFH_EWORKSPACE: return "workspace too small";
+        case PFH_ETOOBIG: return "matrix size beyond this entry point";
+    }
+    return hipGetErrorString((hipError_t)status);
+}
+
+int pfh_pfaffian_max_n(void) { return 64; }
+
+int pfh_pfaffian_f32(const float* A, int64_t batch, int32_t n, float* out, hipStream_t s) {
+    return pfaffian_dense<float>(PFH_F32, A, batch, n, out, s);
+}
+int pfh_pfaffian_f64(const double* A, int64_t batch, int32_t n, double* out, hipStream_t s) {
+    return pfaffian_dense<double>(PFH_F64, A, batch, n, out, s);
+}
+int pfh_pfaffian_c64(const void* A, int64_t batch, int32_t n, void* out, hipStream_t s) {
+    return pfaffian_dense<c64>(PFH_C64, A, batch, n, out, s);
+}
+int pfh_pfaffian_c128(const void* A, int64_t batch, int32_t n, void* out, hipStream_t s) {
+    return pfaffian_dense<c128>(PFH_C128, A, batch, n, out, s);
+}
+
+int pfh_slog_pfaffian_workspace_size(int dtype, int64_t batch, int32_t n, int method, size_t* bytes) {
+    if (!bytes || dtype_bytes(dtype) == 0 || batch < 0 || n < 0) return PFH_EINVAL;
+    *bytes = slog_ws_bytes(dtype, batch, n, method);
+    return 0;
+}
+
+int pfh_slog_pfaffian_method(int dtype, const void* A, int64_t batch, int32_t n, int method,
+                             double* log_out, void* phase_out, void* ws, size_t ws_bytes, hipStream_t s) {
+    switch (dtype) {
+        case PFH_F32: return slog_dense<float>(dtype, (const float*)A, batch, n, method, log_out, (float*)phase_out, ws, ws_bytes, s);
+        case PFH_F64: return slog_dense<double>(dtype, (const double*)A, batch, n, method, log_out, (double*)phase_out, ws, ws_bytes, s);
+        case PFH_C64: return slog_dense<c64>(dtype, (const c64*)A, batch, n, method, log_out, (c64*)phase_out, ws, ws_bytes, s);
+        case PFH_C128: return slog_dense<c128>(dtype, (const c128*)A, batch, n, method, log_out, (c128*)phase_out, ws, ws_bytes, s);
+    }
+    return PFH_EINVAL;
+}
+
+int pfh_slog_pfaffian_f32(const float* A, int64_t batch, int32_t n, double* log_out, float* sign_out,
+                          void* ws, size_t ws_bytes, hipStream_t s) {
+    return pfh_slog_pfaffian_method(PFH_F32, A, batch, n, PFH_METHOD_AUTO, log_out, sign_out, ws, ws_bytes, s);
+}
+int pfh_slog_pfaffian_f64(const double* A, int64_t batch, int32_t n, double* log_out, double* sign_out,
+                          void* ws, size_t ws_bytes, hipStream_t s) {
+    return pfh_slog_pfaffian_method(PFH_F64, A, batch, n, PFH_METHOD_AUTO, log_out, sign_out, ws, ws_bytes, s);
+}
+int pfh_slog_pfaffian_c64(const void* A, int64_t batch, int32_t n, double* log_out, void* phase_out,
+                          void* ws, size_t ws_bytes, hipStream_t s) {
+    return pfh_slog_pfaffian_method(PFH_C64, A, batch, n, PFH_METHOD_AUTO, log_out, phase_out, ws, ws_bytes, s);
+}
+int pfh_slog_pfaffian_c128(const void* A, int64_t batch, int32_t n, double* log_out, void* phase_out,
+                           void* ws, size_t ws_bytes, hipStream_t s) {
+    return pfh_slog_pfaffian_method(PFH_C128, A, batch, n, PFH_METHOD_AUTO, log_out, phase_out, ws, ws_bytes, s);
+}
+
+int pfh_slog_pfaffian_ragged_workspace_size(int dtype, int64_t count, int32_t max_n, size_t* bytes) {
+    if (!bytes || dtype_bytes(dtype) == 0 || count < 0 || max_n < 0) return PFH_EINVAL;
+    *bytes = ragged_layout(dtype, count, max_n).total;
+    return 0;
+}
+
+int pfh_slog_pfaffian_ragged(int dtype, const void* data, const int64_t* offsets, const int32_t* sizes,
+                             int64_t count, int32_t max_n, double* log_out, void* phase_out,
+                             void* ws_in, size_t ws_bytes, hipStream_t s) {
+    if (dtype_bytes(dtype) == 0 || count < 0 || max_n < 0) return PFH_EINVAL;
+    if (count == 0) return 0;
+    if (count > 0x7fffffff) return PFH_EINVAL;
+    if (!data || !offsets || !sizes || !log_out || !phase_out) return PFH_EINVAL;
+    const RaggedLayout L = ragged_layout(dtype, count, max_n);
+    void* ws = nullptr;
+    int err = resolve_ws(ws_in, ws_bytes, L.total, &ws);
+    if (err) return err;
+    switch (dtype) {
+        case PFH_F32: return ragged_typed<float>(dtype, data, offsets, sizes, count, max_n, log_out, phase_out, (char*)ws, L, s);
+        case PFH_F64: return ragged_typed<double>(dtype, data, offsets, sizes, count, max_n, log_out, phase_out, (char*)ws, L, s);
+        case PFH_C64: return ragged_typed<c64>(dtype, data, offsets, sizes, count, max_n, log_out, phase_out, (char*)ws, L, s);
+        case PFH_C128: return ragged_typed<c128>(dtype, data, offsets, sizes, count, max_n, log_out, phase_out, (char*)ws, L, s);
+    }
+    return PFH_EINVAL;
+}
+
+}  // extern "C"

@@ -1,0 +1,161 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle,
+the golden fixtures and the reference's own known-answer tests.
+
+Tolerances: product form rel 1e-12 (f64/c128), log|Pf| abs 1e-9 with exact
+sign (f64/c128); float32/complex64 are checked at rel 1e-3 / abs 1e-3
+against the FP64 oracle."""
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def pf(cuda):
+    import pfcuda_amd
+    return pfcuda_amd
+
+
+def unpack(data, sizes):
+    off = 0
+    for n in sizes:
+        yield data[off:off + n * n].reshape(n, n)
+        off += n * n
+
+
+def test_c1_batch_1024x32_f64(pf, cuda):
+    # Plain method, small: 1024 x (32x32) f64, A = (B - B^T)/2, rel 1e-12
+    rng = np.random.default_rng(0)
+    A = oracle.skew_gaussian(rng, 32, batch=1024)
+    got = pf.pfaffian(torch.from_numpy(A).to(cuda)).cpu().numpy()
+    want = np.array([oracle.pf_f64(a)[0] for a in A])
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128, torch.float32, torch.complex64])
+def test_pfaffian_all_sizes(pf, cuda, dtype):
+    rng = np.random.default_rng(1)
+    cplx = dtype.is_complex
+    lo = dtype in (torch.float32, torch.complex64)
+    for n in range(6, 65, 2):
+        A = oracle.skew_gaussian(rng, n, complex_=cplx, batch=5)
+        got = pf.pfaffian(torch.from_numpy(A).to(dtype).to(cuda)).cpu().numpy()
+        for a, g in zip(A, got):
+            want = oracle.pf_c128(a)[0] if cplx else oracle.pf_f64(a)[0]
+            if lo:
+                assert abs(g - want) <= 1e-3 * abs(want) + 1e-5, (n, g, want)
+            else:
+                assert abs(g - want) <= 1e-12 * abs(want) * max(1.0, n / 16), (n, g, want)
+
+
+def test_pfaffian_golden_reference_values(pf, cuda):
+    g = np.load(os.path.join(GOLD, "ref_product.npz"))
+    for A, want in zip(unpack(g["data"], g["sizes"]), g["pf"]):
+        got = float(pf.pfaffian(torch.from_numpy(A.copy()).to(cuda)).item())
+        assert got == pytest.approx(want, rel=1e-12, abs=1e-300)
+
+
+def test_reference_known_answers_pfaffian(pf, cuda):
+    # test/pfaffian_test.py: zero matrix, odd dims, 2x2, det relation, identity
+    for n in range(2, 20, 2):
+        assert float(pf.pfaffian(torch.zeros(n, n, dtype=torch.float64, device=cuda))) == 0.0
+    rng = np.random.default_rng(0)
+    for n in [1, 3, 5, 11, 15]:
+        B = rng.normal(size=(n, n))
+        assert abs(float(pf.pfaffian(torch.from_numpy(B - B.T).to(cuda)))) <= 1e-12
+    for a in [0.0, 1.0, -3.5, 1e-12, 1e10]:
+        A = torch.tensor([[0, a], [-a, 0]], dtype=torch.float64, device=cuda)
+        assert float(pf.pfaffian(A)) == pytest.approx(a)
+    rng = np.random.default_rng(1)
+    for n in range(2, 33, 2):
+        B = rng.normal(size=(n, n))
+        A = B - B.T
+        p = float(pf.pfaffian(torch.from_numpy(A).to(cuda)))
+        assert np.isclose(p * p, np.linalg.det(A), rtol=1e-8, atol=1e-10)
+    rng = np.random.default_rng(seed=0)
+    for n in range(2, 33, 2):
+        for _ in range(10):
+            M, want = oracle.tridiag_identity_case(rng, n)
+            got = float(pf.pfaffian(torch.from_numpy(M).to(cuda)))
+            assert np.isclose(want, got, rtol=1e-10, atol=1e-12), n
+
+
+def test_reference_known_answers_slog(pf, cuda):
+    # test/slog_pfaffian_test.py: zero, odd, det relation, identity
+    for n in range(34, 68, 2):
+        la, s = pf.slog_pfaffian(torch.zeros(n, n, dtype=torch.float64, device=cuda))
+        assert float(s) == 0 and math.isinf(float(la)) and float(la) < 0
+    rng = np.random.default_rng(0)
+    for n in [35, 37, 39]:
+        B = rng.normal(size=(n, n))
+        la, s = pf.slog_pfaffian(torch.from_numpy(B - B.T).to(cuda))
+        assert float(s) == 0 and float(la) == -math.inf
+    rng = np.random.default_rng(1)
+    for n in range(34, 66, 2):
+        B = rng.normal(size=(n, n))
+        A = B - B.T
+        la, s = pf.slog_pfaffian(torch.from_numpy(A).to(cuda))
+        assert np.isclose(np.exp(2 * float(la)), np.linalg.det(A), rtol=1e-8, atol=1e-10)
+    rng = np.random.default_rng(seed=0)
+    for n in range(40, 300, 10):
+        for _ in range(10):
+            M, want = oracle.tridiag_identity_case(rng, n)
+            la, s = pf.slog_pfaffian(torch.from_numpy(M).to(cuda))
+            got = float(s) * math.exp(float(la))
+            assert np.isclose(want, got, rtol=1e-10, atol=1e-12), n
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_slog_vs_oracle_and_mpmath(pf, cuda, cplx):
+    kind = "complex" if cplx else "real"
+    g = np.load(os.path.join(GOLD, "mp_slog.npz"))
+    for A, la, ph in zip(unpack(g[f"{kind}_data"], g[f"{kind}_sizes"]), g[f"{kind}_log"], g[f"{kind}_phase"]):
+        gl, gp = pf.slog_pfaffian_batched(torch.from_numpy(A.copy()).to(cuda))
+        assert abs(float(gl) - la) < 1e-10
+        assert abs(complex(gp.item()) - ph) < 1e-9
+    rng = np.random.default_rng(2)
+    for n in (34, 64, 100, 128, 130, 192, 200, 256):
+        A = oracle.skew_gaussian(rng, n, complex_=cplx, batch=3)
+        gl, gp = pf.slog_pfaffian(torch.from_numpy(A).to(cuda))
+        for a, l1, p1 in zip(A, gl.cpu().numpy(), gp.cpu().numpy()):
+            l0, p0 = oracle.slog(a)
+            assert abs(l1 - l0) < 1e-9, (n, l1, l0)
+            assert abs(p1 - p0) < 1e-9, (n, p1, p0)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
+def test_ragged_vs_oracle(pf, cuda, dtype):
+    rng = np.random.default_rng(3)
+    cplx = dtype.is_complex
+    sizes = [int(x) for x in 2 * rng.integers(8, 129, size=300)] + [2, 4, 0, 7, 33, 256, 256, 40]
+    mats = [oracle.skew_gaussian(rng, n, complex_=cplx) for n in sizes]
+    mats[-1][:] = 0                                   # a zero matrix: (-inf, 0)
+    data = np.concatenate([m.reshape(-1) for m in mats])
+    offs = np.cumsum([0] + [int(n) * int(n) for n in sizes[:-1]])
+    gl, gp = pf.slog_pfaffian_ragged(torch.from_numpy(data).to(cuda), offs, sizes)
+    gl, gp = gl.cpu().numpy(), gp.cpu().numpy()
+    wl, wp = oracle.slog_ragged(data, offs, sizes, threads=8)
+    for i, n in enumerate(sizes):
+        if math.isinf(wl[i]):
+            assert math.isinf(gl[i]) and gp[i] == 0, (i, n)
+        else:
+            assert abs(gl[i] - wl[i]) < 1e-9, (i, n, gl[i], wl[i])
+            assert abs(gp[i] - wp[i]) < 1e-9, (i, n, gp[i], wp[i])
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
+@pytest.mark.parametrize("n", [300, 512])
+def test_unblocked_large_vs_oracle(pf, cuda, dtype, n):
+    rng = np.random.default_rng(n)
+    A = oracle.skew_gaussian(rng, n, complex_=dtype.is_complex)
+    gl, gp = pf.slog_pfaffian(torch.from_numpy(A).to(cuda), method=pf.PFH_METHOD_UNBLOCKED)
+    l0, p0 = oracle.slog(A)
+    assert abs(float(gl) - l0) < 1e-9
+    assert abs(complex(gp.item()) - p0) < 1e-9
