@@ -6,6 +6,7 @@
 #include "../../include/pfaffian_hip.h"
 #include "pf_batch.h"
 #include "pf_large.h"
+#include "pf_blocked.h"
 
 using namespace pfh;
 
@@ -223,8 +224,10 @@ int pfaffian_dense(int dt, const void* A, int64_t batch, int32_t n, void* out, h
 
 size_t slog_ws_bytes(int dt, int64_t batch, int32_t n, int method) {
     if (n <= reg_max_n(dt) && method == PFH_METHOD_AUTO) return 0;
-    if (n <= reg_max_n(dt) && method != PFH_METHOD_AUTO) { /* forced large path */ }
     const size_t es = dtype_bytes(dt);
+    if (method == PFH_METHOD_BLOCKED || method == PFH_METHOD_MIXED ||
+        (method == PFH_METHOD_AUTO && n > reg_max_n(dt)))
+        return blocked_workspace_bytes(dt, n, method == PFH_METHOD_MIXED);
     return align_up(es * (size_t)n * n) + align_up(sizeof(LargeState<c128>));
 }
 
@@ -250,12 +253,17 @@ int slog_dense(int dt, const T* A, int64_t batch, int32_t n, int method, double*
     const size_t need = slog_ws_bytes(dt, batch, n, method);
     int err = resolve_ws(ws_in, ws_bytes, need, &ws);
     if (err) return err;
-    if (method == PFH_METHOD_BLOCKED || method == PFH_METHOD_MIXED) return PFH_EINVAL;
+    const bool blocked = (method != PFH_METHOD_UNBLOCKED);
+    const int nb = blocked_panel_width(dt);
     for (int64_t b = 0; b < batch; ++b) {
         const T* Ab = A + b * (int64_t)n * n;
-        T* W = (T*)ws;
-        LargeState<T>* st = (LargeState<T>*)((char*)ws + align_up(sizeof(T) * (size_t)n * n));
-        err = large_slog<T>(Ab, n, W, st, log_out + b, phase_out + b, s);
+        if (blocked) {
+            err = blocked_slog<T>(Ab, n, nb, method == PFH_METHOD_MIXED, ws, log_out + b, phase_out + b, s);
+        } else {
+            T* W = (T*)ws;
+            LargeState<T>* st = (LargeState<T>*)((char*)ws + align_up(sizeof(T) * (size_t)n * n));
+            err = large_slog<T>(Ab, n, W, st, log_out + b, phase_out + b, s);
+        }
         if (err) return err;
     }
     return 0;

@@ -10,7 +10,8 @@
 //   thread t holds physical column c = t % NCOL, row block rb = t / NCOL;
 //   logical row r >= LR lives in register (r-LR)/NRB of block (r-LR)%NRB
 //   (rows interleaved so that the register index of a row is wave-uniform);
-//   rows r < LR live in LDS.  Rows are swapped physically; columns carry a
+//   rows r < LR live in LDS, row r owned by block r % NRB.  Rows are swapped
+//   physically (each by its owning threads); columns carry a
 //   logical label L(c) instead, so a column swap costs nothing.
 //
 // Per pair step k (right-looking, as pf_sm_calc :29-91):
@@ -101,7 +102,7 @@ pf_reg_kernel(BatchDesc d) {
             T x = zero<T>();
             bool ownk;
             if (k < LR) {
-                ownk = (rb == 0);
+                ownk = (rb == k % NRB);
                 if (ownk) { x = Ls[k][c]; Xs[L] = x; }
             } else {
                 const int kk = k - LR;
@@ -134,9 +135,9 @@ pf_reg_kernel(BatchDesc d) {
             const int r1 = k + 1;
             T v1 = zero<T>(), vp = zero<T>();
             bool own1, ownp;
-            if (r1 < LR) { own1 = (rb == 0); if (own1) v1 = Ls[r1][c]; }
+            if (r1 < LR) { own1 = (rb == r1 % NRB); if (own1) v1 = Ls[r1][c]; }
             else { own1 = (rb == (r1 - LR) % NRB); if (own1) v1 = a.dget((r1 - LR) / NRB); }
-            if (p < LR) { ownp = (rb == 0); if (ownp) vp = Ls[p][c]; }
+            if (p < LR) { ownp = (rb == p % NRB); if (ownp) vp = Ls[p][c]; }
             else { ownp = (rb == (p - LR) % NRB); if (ownp) vp = a.dget((p - LR) / NRB); }
             int Lnew = L;
             if (sw) { if (L == r1) Lnew = p; else if (L == p) Lnew = r1; }
@@ -163,8 +164,11 @@ pf_reg_kernel(BatchDesc d) {
             // 4. rank-2 update of rows >= k+2
             const T inv = recip(piv);
             const T xs = xc * inv, ys = yc * inv;
+            // LDS row r always belongs to thread block r % NRB, the same thread
+            // that wrote it in the interchange above, so no barrier is needed.
             if (LR > 0 && k + 2 < LR) {
-                for (int r = k + 2 + rb; r < LR && r < n; r += NRB) {
+                const int r_first = k + 2 + ((rb - (k + 2)) % NRB + NRB) % NRB;
+                for (int r = r_first; r < LR && r < n; r += NRB) {
                     const XY q = XYs[r];
                     Ls[r][c] = mul_add_update(Ls[r][c], q.y, xs, q.x, ys);
                 }

@@ -159,3 +159,25 @@ def test_unblocked_large_vs_oracle(pf, cuda, dtype, n):
     l0, p0 = oracle.slog(A)
     assert abs(float(gl) - l0) < 1e-9
     assert abs(complex(gp.item()) - p0) < 1e-9
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
+@pytest.mark.parametrize("n", [258, 300, 640, 1000])
+def test_blocked_large_vs_oracle(pf, cuda, dtype, n):
+    rng = np.random.default_rng(n + 1)
+    A = oracle.skew_gaussian(rng, n, complex_=dtype.is_complex)
+    gl, gp = pf.slog_pfaffian(torch.from_numpy(A).to(cuda), method=pf.PFH_METHOD_BLOCKED)
+    l0, p0 = oracle.slog(A)
+    assert abs(float(gl) - l0) < 1e-9, (float(gl), l0)
+    assert abs(complex(gp.item()) - p0) < 1e-9
+
+
+def test_blocked_zero_and_identity(pf, cuda):
+    la, s = pf.slog_pfaffian(torch.zeros(300, 300, dtype=torch.float64, device=cuda),
+                             method=pf.PFH_METHOD_BLOCKED)
+    assert float(s) == 0 and float(la) == -math.inf
+    rng = np.random.default_rng(seed=0)
+    for n in (260, 290, 330):
+        M, want = oracle.tridiag_identity_case(rng, n)
+        la, s = pf.slog_pfaffian(torch.from_numpy(M).to(cuda), method=pf.PFH_METHOD_BLOCKED)
+        assert np.isclose(want, float(s) * math.exp(float(la)), rtol=1e-10, atol=1e-12), n
