@@ -94,6 +94,19 @@ int pfh_slog_pfaffian_ragged(int dtype, const void* data, const int64_t* offsets
                              int64_t count, int32_t max_n, double* log_out, void* phase_out,
                              void* workspace, size_t workspace_bytes, hipStream_t stream);
 
+/* Workspace bytes for pfh_slog_pfaffian_grad. */
+int pfh_slog_pfaffian_grad_workspace_size(int dtype, int64_t batch, int32_t n, size_t* bytes);
+
+/* log|Pf|, sign/phase and the gradient G = d log Pf / dA = 1/2 A^-T of
+ * `batch` matrices of size n, from one block-sweep pass (Gauss-Jordan with
+ * the forward pass's 2x2 pivots), so the factorization is recomputed rather
+ * than stored.  grad_out has the input's shape and dtype.  Replaces the
+ * reference's reverse-mode path, which differentiates its JVP rule
+ * (pfcuda/cuda_api.py:93-107) through jnp.linalg.inv (:101). */
+int pfh_slog_pfaffian_grad(int dtype, const void* A, int64_t batch, int32_t n, double* log_out,
+                           void* phase_out, void* grad_out, void* workspace, size_t workspace_bytes,
+                           hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

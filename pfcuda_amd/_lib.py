@@ -27,7 +27,7 @@ class PfaffianLibError(RuntimeError):
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
-            f"{LIB_PATH} is not built; run `python -m pfcuda_amd.build` (hipcc, gfx950)")
+            f"{LIB_PATH} is not built; run `python pfcuda_amd/build.py` (hipcc, gfx950)")
     lib = ctypes.CDLL(LIB_PATH)
     lib.pfh_status_string.argtypes = [ctypes.c_int]
     lib.pfh_status_string.restype = ctypes.c_char_p
@@ -41,6 +41,8 @@ def _load():
     lib.pfh_slog_pfaffian_method.argtypes = [ctypes.c_int, _vp, _i64, _i32, ctypes.c_int, _vp, _vp, _vp, _sz, _vp]
     lib.pfh_slog_pfaffian_ragged_workspace_size.argtypes = [ctypes.c_int, _i64, _i32, ctypes.POINTER(_sz)]
     lib.pfh_slog_pfaffian_ragged.argtypes = [ctypes.c_int, _vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _sz, _vp]
+    lib.pfh_slog_pfaffian_grad_workspace_size.argtypes = [ctypes.c_int, _i64, _i32, ctypes.POINTER(_sz)]
+    lib.pfh_slog_pfaffian_grad.argtypes = [ctypes.c_int, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _sz, _vp]
     return lib
 
 

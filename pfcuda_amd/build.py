@@ -2,7 +2,8 @@
 
 Each .hip translation unit is compiled to an object (in parallel), then
 linked into one shared library next to this file.  Run as
-``python -m pfcuda_amd.build`` or through ``__graft_entry__.build()``.
+``python pfcuda_amd/build.py`` (by path, so that the package, which loads
+the library, is not imported) or through ``__graft_entry__.build()``.
 """
 from __future__ import annotations
 

@@ -22,13 +22,16 @@ struct BatchDesc {
     void* prod_out;           // MODE_PROD: Pf as the input type
     double* log_out;          // MODE_SLOG: log|Pf|
     void* phase_out;          // MODE_SLOG: sign / unit phase as the input type
+    void* grad_out;           // MODE_GRAD: 1/2 A^-T, laid out like the input
 };
 
-enum { MODE_PROD = 0, MODE_SLOG = 1 };
+enum { MODE_PROD = 0, MODE_SLOG = 1, MODE_GRAD = 2 };
 enum { DT_F32 = 0, DT_F64 = 1, DT_C64 = 2, DT_C128 = 3 };
 
-// Largest n the register-resident workgroup kernels take for a dtype.
+// Largest n the register-resident workgroup kernels take for a dtype
+// (MODE_GRAD: reg_grad_max_n).
 int reg_max_n(int dtype);
+int reg_grad_max_n(int dtype);
 
 // Launches the workgroup kernel whose capacity covers `maxn` for every
 // matrix in the batch.  Returns 0 or a hipError_t value.

@@ -269,9 +269,61 @@ int slog_dense(int dt, const T* A, int64_t batch, int32_t n, int method, double*
     return 0;
 }
 
+size_t grad_ws_bytes(int dt, int32_t n) {
+    if (n <= reg_grad_max_n(dt)) return 0;
+    return align_up(dtype_bytes(dt) * (size_t)n * n) + align_up(sizeof(int) * (size_t)n) +
+           align_up(sizeof(LargeState<c128>));
+}
+
+template <typename T>
+int grad_dense(int dt, const T* A, int64_t batch, int32_t n, double* log_out, T* phase_out, T* G,
+               void* ws_in, size_t ws_bytes, hipStream_t s) {
+    if (check_dense(batch, n)) return PFH_EINVAL;
+    if (batch == 0) return 0;
+    if (!A || !log_out || !phase_out || !G) return PFH_EINVAL;
+    if (n <= reg_grad_max_n(dt) || (n & 1)) {
+        if (n > reg_grad_max_n(dt)) return PFH_ETOOBIG;   // odd and too large: nothing to sweep
+        BatchDesc d{};
+        d.data = A; d.count = batch; d.dense_n = n; d.log_out = log_out; d.phase_out = phase_out;
+        d.grad_out = G;
+        return launch_reg(dt, MODE_GRAD, d, n < 2 ? 2 : n, s);
+    }
+    void* ws = nullptr;
+    int err = resolve_ws(ws_in, ws_bytes, grad_ws_bytes(dt, n), &ws);
+    if (err) return err;
+    char* b = (char*)ws;
+    T* W = (T*)b;
+    int* perm = (int*)(b + align_up(sizeof(T) * (size_t)n * n));
+    LargeState<T>* st = (LargeState<T>*)(b + align_up(sizeof(T) * (size_t)n * n) + align_up(sizeof(int) * (size_t)n));
+    for (int64_t i = 0; i < batch; ++i) {
+        const int64_t off = i * (int64_t)n * n;
+        err = large_grad<T>(A + off, n, W, perm, st, log_out + i, phase_out + i, G + off, s);
+        if (err) return err;
+    }
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int pfh_slog_pfaffian_grad_workspace_size(int dtype, int64_t batch, int32_t n, size_t* bytes) {
+    if (!bytes || dtype_bytes(dtype) == 0 || batch < 0 || n < 0) return PFH_EINVAL;
+    *bytes = grad_ws_bytes(dtype, n);
+    return 0;
+}
+
+int pfh_slog_pfaffian_grad(int dtype, const void* A, int64_t batch, int32_t n, double* log_out, void* phase_out,
+                           void* grad_out, void* ws, size_t ws_bytes, hipStream_t s) {
+    switch (dtype) {
+        case PFH_F32: return grad_dense<float>(dtype, (const float*)A, batch, n, log_out, (float*)phase_out, (float*)grad_out, ws, ws_bytes, s);
+        case PFH_F64: return grad_dense<double>(dtype, (const double*)A, batch, n, log_out, (double*)phase_out, (double*)grad_out, ws, ws_bytes, s);
+        case PFH_C64: return grad_dense<c64>(dtype, (const c64*)A, batch, n, log_out, (c64*)phase_out, (c64*)grad_out, ws, ws_bytes, s);
+        case PFH_C128: return grad_dense<c128>(dtype, (const c128*)A, batch, n, log_out, (c128*)phase_out, (c128*)grad_out, ws, ws_bytes, s);
+    }
+    return PFH_EINVAL;
+}
+
 
 const char* pfh_status_string(int status) {
     switch (status) {

@@ -21,6 +21,12 @@ struct LargeState {
 template <typename T>
 int large_slog(const T* A, int64_t n, T* W, LargeState<T>* st, double* log_out, T* phase_out, hipStream_t s);
 
+// log|Pf|, sign/phase and G = 1/2 A^-T of one n x n matrix (block sweep).
+// W: n*n scratch, perm: n ints, st: state.
+template <typename T>
+int large_grad(const T* A, int64_t n, T* W, int* perm, LargeState<T>* st, double* log_out, T* phase_out, T* G,
+               hipStream_t s);
+
 // One workgroup per matrix over per-workgroup slabs of slab_elems elements.
 template <typename T>
 int launch_glb(const BatchDesc& d, T* slabs, int64_t slab_elems, int grid, hipStream_t s);

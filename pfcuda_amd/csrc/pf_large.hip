@@ -168,6 +168,111 @@ __global__ void __launch_bounds__(64) lg_finish(const LargeState<T>* st, double*
     *phase_out = ph;
 }
 
+// ---------------------------------------------------------------------------
+// Gradient of log|Pf| for matrices past the register kernels: the same
+// block sweep as the register kernel's MODE_GRAD (swept rows stored negated,
+// so the working matrix stays skew and one rank-2 update serves every row),
+// over a full-square working copy with physical interchanges of whole rows
+// and columns.  Per pair step: lg_pivot, gs_swap, gs_update (every row and
+// column outside the pair), gs_rows (the pair's two rows).  At the end the
+// working matrix is A^-1 in pivot order; gs_out writes -1/2 A^-1 = 1/2 A^-T
+// back in the caller's order.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(1024) gs_swap(T* __restrict__ W, int* __restrict__ perm, int64_t n, int k,
+                                                const LargeState<T>* st) {
+    if (st->dead) return;
+    const int p = st->p, i = k + 1;
+    if (p == i) return;
+    for (int64_t c = threadIdx.x; c < n; c += blockDim.x) {
+        T a = W[i * n + c], b = W[(int64_t)p * n + c];
+        W[i * n + c] = b; W[(int64_t)p * n + c] = a;
+    }
+    __syncthreads();
+    for (int64_t r = threadIdx.x; r < n; r += blockDim.x) {
+        T a = W[r * n + i], b = W[r * n + p];
+        W[r * n + i] = b; W[r * n + p] = a;
+    }
+    if (threadIdx.x == 0) { int t = perm[i]; perm[i] = perm[p]; perm[p] = t; }
+}
+
+// rows r, columns c outside the pair {k, k+1}; columns k, k+1 get -y/a, x/a
+template <typename T, int RPT>
+__global__ void __launch_bounds__(256) gs_update(T* __restrict__ W, int64_t n, int k, const LargeState<T>* st) {
+    if (st->dead) return;
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int r0 = blockIdx.y * (4 * RPT) + (threadIdx.x >> 6);
+    if (c >= n) return;
+    const T inv = recip(st->piv);
+    const T* X = W + (int64_t)k * n;
+    const T* Y = W + (int64_t)(k + 1) * n;
+    T xs, ys;
+    if (c == k) { xs = -inv; ys = zero<T>(); }
+    else if (c == k + 1) { xs = zero<T>(); ys = -inv; }
+    else { xs = X[c] * inv; ys = Y[c] * inv; }
+    const bool special = (c == k || c == k + 1);
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+        const int r = r0 + 4 * i;
+        if (r < n && r != k && r != k + 1) {
+            T* e = W + (int64_t)r * n + c;
+            const T base = special ? zero<T>() : *e;
+            *e = base + Y[r] * xs - X[r] * ys;
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) gs_rows(T* __restrict__ W, int64_t n, int k, const LargeState<T>* st) {
+    if (st->dead) return;
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const T inv = recip(st->piv);
+    T* X = W + (int64_t)k * n;
+    T* Y = W + (int64_t)(k + 1) * n;
+    const T x = X[c], y = Y[c];
+    if (c == k) { X[c] = zero<T>(); Y[c] = inv; }
+    else if (c == k + 1) { X[c] = -inv; Y[c] = zero<T>(); }
+    else { X[c] = y * inv; Y[c] = -(x * inv); }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) gs_out(const T* __restrict__ W, const int* __restrict__ perm, int64_t n,
+                                               const LargeState<T>* st, T* __restrict__ G) {
+    using R = typename real_of<T>::type;
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= n * n) return;
+    const int64_t r = e / n, c = e % n;
+    const T v = st->dead ? nan_of<T>() : W[e] * (R)(-0.5);
+    G[(int64_t)perm[r] * n + perm[c]] = v;
+}
+
+template <typename T>
+__global__ void gs_init_perm(int* perm, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        perm[i] = (int)i;
+}
+
+template <typename T>
+int large_grad(const T* A, int64_t n, T* W, int* perm, LargeState<T>* st, double* log_out, T* phase_out, T* G,
+               hipStream_t s) {
+    hipError_t e = hipMemcpyAsync(W, A, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(lg_init<T>, dim3(1), dim3(64), 0, s, st);
+    hipLaunchKernelGGL(gs_init_perm<T>, dim3(64), dim3(256), 0, s, perm, n);
+    constexpr int RPT = 8;
+    const dim3 grid((unsigned)((n + 63) / 64), (unsigned)((n + 4 * RPT - 1) / (4 * RPT)));
+    for (int k = 0; k + 1 < n; k += 2) {
+        hipLaunchKernelGGL(lg_pivot<T>, dim3(1), dim3(1024), 0, s, W, n, k, st);
+        hipLaunchKernelGGL(gs_swap<T>, dim3(1), dim3(1024), 0, s, W, perm, n, k, st);
+        hipLaunchKernelGGL((gs_update<T, RPT>), grid, dim3(256), 0, s, W, n, k, st);
+        hipLaunchKernelGGL(gs_rows<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, W, n, k, st);
+    }
+    hipLaunchKernelGGL(lg_finish<T>, dim3(1), dim3(64), 0, s, st, log_out, phase_out);
+    hipLaunchKernelGGL(gs_out<T>, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, s, W, perm, n, st, G);
+    return (int)hipGetLastError();
+}
+
 template <typename T>
 int large_slog(const T* A, int64_t n, T* W, LargeState<T>* st, double* log_out, T* phase_out, hipStream_t s) {
     hipError_t e = hipMemcpyAsync(W, A, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s);
@@ -269,6 +374,7 @@ int launch_glb(const BatchDesc& d, T* slabs, int64_t slab_elems, int grid, hipSt
 
 #define PFH_INST(T)                                                                          \
     template int large_slog<T>(const T*, int64_t, T*, LargeState<T>*, double*, T*, hipStream_t); \
+    template int large_grad<T>(const T*, int64_t, T*, int*, LargeState<T>*, double*, T*, T*, hipStream_t); \
     template int launch_glb<T>(const BatchDesc&, T*, int64_t, int, hipStream_t);
 PFH_INST(float)
 PFH_INST(double)

@@ -8,24 +8,40 @@
 //
 // Layout (NCOL column lanes x NRB row blocks = threads):
 //   thread t holds physical column c = t % NCOL, row block rb = t / NCOL;
-//   logical row r >= LR lives in register (r-LR)/NRB of block (r-LR)%NRB
-//   (rows interleaved so that the register index of a row is wave-uniform);
-//   rows r < LR live in LDS, row r owned by block r % NRB.  Rows are swapped
-//   physically (each by its owning threads); columns carry a
-//   logical label L(c) instead, so a column swap costs nothing.
+//   physical row phi >= LR lives in register (phi-LR)/NRB of block
+//   (phi-LR) % NRB (interleaved, so the register index of a row is
+//   wave-uniform); rows phi < LR live in LDS, owned by block phi % NRB.
+// Pivoting is virtual: nothing moves.  lab[phi] is the logical position of
+// physical index phi (rows and columns alike, the permutation being
+// symmetric) and pos[] its inverse, as pf_sm_calc keeps P[] (:11-12, :48-55).
+// Registers are only ever written with compile-time indices; the two rows a
+// step needs are read with wave-uniform indices (scalar branches).
 //
 // Per pair step k (right-looking, as pf_sm_calc :29-91):
-//   1. the owners of row k write it to X[L(c)]             (barrier B1)
-//   2. every wave reduces argmax_{j>k} |X[j]| (ties: lowest j) -> p, pivot
-//   3. rows k+1 and p swap storage slots, labels k+1 <-> p swap; the new
-//      row k+1 goes to Y[L(c)]                             (barrier B2)
-//   4. rank-2 Schur update of rows r >= k+2 in every column:
-//        A[r][c] += (Y[r] * X[c] - X[r] * Y[c]) / pivot
-//      (the same update as pfaffian_sm.cu:71-89, written for the skew
-//      Schur complement S = A22 + (y x^T - x y^T)/a).
-// Lanes whose column is already eliminated compute values nobody reads, so
-// the update needs no column mask.  Pf = (-1)^swaps * prod(pivots); the
-// exact zero-pivot test returns Pf = 0 (log -inf, phase 0).
+//   1. f = pos[k]; the owners of row f write it to X[c]          (barrier B1)
+//   2. every wave reduces argmax |X[c]| over columns with lab[c] in (k, n),
+//      ties to the lowest logical index -> q, pivot a = X[q]
+//   3. the owners of row q write (X[c], Y[c] = A[q][c]) to XY[c]  (barrier B2);
+//      thread 0 then swaps logical positions k+1 <-> lab[q]
+//   4. rank-2 Schur update of every row phi, every column c:
+//        A[phi][c] += (Y[phi] X[c] - X[phi] Y[c]) / a
+//      (the update of pfaffian_sm.cu:71-89, written for the skew Schur
+//      complement S = A22 + (y x^T - x y^T)/a).
+// Rows and columns already eliminated take values nobody reads again, so the
+// update has no masks or branches.  Pf = (-1)^swaps * prod(pivots); the exact
+// zero-pivot test gives Pf = 0 (log -inf, phase 0).
+//
+// MODE_GRAD turns the same pass into a block sweep (Gauss-Jordan with the
+// 2x2 Parlett-Reid pivots).  Sweeping S = {f, q}, B = [[0, a], [-a, 0]],
+// maps A[S][j] -> B^-1 A[S][j], A[i][S] -> A[i][S] B^-1, A[S][S] -> -B^-1 and
+// A[i][j] -> A[i][j] - A[i][S] B^-1 A[S][j]; sweeping every pair leaves -A^-1.
+// Swept rows are kept negated, which keeps the whole working matrix skew, so
+// the one update above serves every row once two entries are changed:
+// XY[f] = (-1, -a), XY[q] = (a, -1) (rows f, q become y/a and -x/a) and
+// lanes f, q use (X, Y) coefficients (-1/a, -1) and (1, -1/a) (columns f, q
+// become -y/a and x/a).  At the end the matrix is A^-1 in the caller's
+// order, and the gradient of log|Pf| (the reference's JVP,
+// pfcuda/cuda_api.py:93-107, transposed) is 1/2 A^-T = -1/2 A^-1.
 #include "pf_batch.h"
 #include "pf_regstore.h"
 
@@ -43,9 +59,10 @@ pf_reg_kernel(BatchDesc d) {
     constexpr int RB = (NCOL - LR) / NRB;
     using R = typename real_of<T>::type;
     struct XY { T x, y; };
-    __shared__ T Xs[NCOL];      // row k by old labels (pivot search)
-    __shared__ XY XYs[NCOL];    // rows k, k+1 by new labels (update)
-    __shared__ T EXs[NCOL];
+    __shared__ T Xs[NCOL];       // pivot row f, by physical column
+    __shared__ XY XYs[NCOL];     // rows f and q, by physical index
+    __shared__ int lab[NCOL];    // physical -> logical
+    __shared__ int pos[NCOL];    // logical -> physical
     __shared__ T Ls[LR > 0 ? LR : 1][NCOL];
 
     const int t = threadIdx.x;
@@ -61,7 +78,8 @@ pf_reg_kernel(BatchDesc d) {
     for (int64_t j = blockIdx.x; j < count; j += gridDim.x) {
         const int idx = d.order ? d.order[(d.begin_dev ? *d.begin_dev : 0) + j] : (int)j;
         const int n = __builtin_amdgcn_readfirstlane(d.sizes ? d.sizes[idx] : d.dense_n);
-        const T* base = (const T*)d.data + (d.offsets ? d.offsets[idx] : (int64_t)idx * n * n);
+        const int64_t off = d.offsets ? d.offsets[idx] : (int64_t)idx * n * n;
+        const T* base = (const T*)d.data + off;
         __syncthreads();   // LDS of the previous matrix is free
 
         if (n < 0 || n > NCOL) {   // size outside this kernel's capacity: NaN, no access
@@ -77,6 +95,10 @@ pf_reg_kernel(BatchDesc d) {
                 if (MODE == MODE_PROD) ((T*)d.prod_out)[idx] = zero<T>();
                 else { d.log_out[idx] = -INFINITY; ((T*)d.phase_out)[idx] = zero<T>(); }
             }
+            if constexpr (MODE == MODE_GRAD) {   // singular: no gradient
+                T* G = (T*)d.grad_out + off;
+                for (int e = t; e < n * n; e += NCOL * NRB) G[e] = nan_of<T>();
+            }
             continue;
         }
 
@@ -90,110 +112,101 @@ pf_reg_kernel(BatchDesc d) {
             for (int r = rb; r < LR; r += NRB)
                 Ls[r][c] = (r < n && c < n) ? base[(int64_t)r * n + c] : zero<T>();
         }
+        if (rb == 0) { lab[c] = c; pos[c] = c; }
+        __syncthreads();
 
-        int L = c;
         T prod = one<T>();
         LogAcc la;
         Phase<T> ph;
         bool dead = false;
+        int f = 0;   // physical index of logical row k
 
         for (int k = 0; k < n; k += 2) {
-            // 1. row k -> X
-            T x = zero<T>();
-            bool ownk;
-            if (k < LR) {
-                ownk = (rb == k % NRB);
-                if (ownk) { x = Ls[k][c]; Xs[L] = x; }
+            // 1. row f -> X
+            if (f < LR) {
+                if (rb == f % NRB) Xs[c] = Ls[f][c];
             } else {
-                const int kk = k - LR;
-                ownk = (rb == kk % NRB);
-                if (ownk) { x = a.dget(kk / NRB); Xs[L] = x; }
+                if (rb == (f - LR) % NRB) Xs[c] = a.dget((f - LR) / NRB);
             }
             __syncthreads();   // B1
 
             // 2. pivot search over logical columns k+1 .. n-1
             double best = -1.0;
-            int bi = 0x7fffffff;
-            for (int jj = k + 1 + lane; jj < n; jj += 64) {
-                const double m = (double)mag2(Xs[jj]);
-                if (m > best) { best = m; bi = jj; }
+            int bl = 0x7fffffff;
+            for (int cc = lane; cc < NCOL; cc += 64) {
+                const int l = lab[cc];
+                if (l > k && l < n) {
+                    const double m = (double)mag2(Xs[cc]);
+                    if (m > best || (m == best && l < bl)) { best = m; bl = l; }
+                }
             }
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ob = __shfl_xor(best, off);
-                const int oi = __shfl_xor(bi, off);
-                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            for (int o = 32; o > 0; o >>= 1) {
+                const double ob = __shfl_xor(best, o);
+                const int ol = __shfl_xor(bl, o);
+                if (ob > best || (ob == best && ol < bl)) { best = ob; bl = ol; }
             }
-            int p = __builtin_amdgcn_readfirstlane(bi);
-            if (p >= n) p = k + 1;              // only NaN candidates: keep k+1
-            const T piv = Xs[p];
-            const T xc = Xs[L];
+            int lq = __builtin_amdgcn_readfirstlane(bl);
+            if (lq >= n) lq = k + 1;              // only NaN candidates: keep k+1
+            const int q = __builtin_amdgcn_readfirstlane(pos[lq]);
+            const T piv = Xs[q];
             if (is_zero(piv)) { dead = true; break; }
-            const bool sw = (p != k + 1);
+            const bool sw = (lq != k + 1);
+            const int p1 = pos[k + 1];
+            const int f_next = (k + 2 < n) ? ((lq == k + 2) ? p1 : pos[k + 2]) : 0;
 
-            // 3. swap rows k+1 <-> p, relabel columns
-            const int r1 = k + 1;
-            T v1 = zero<T>(), vp = zero<T>();
-            bool own1, ownp;
-            if (r1 < LR) { own1 = (rb == r1 % NRB); if (own1) v1 = Ls[r1][c]; }
-            else { own1 = (rb == (r1 - LR) % NRB); if (own1) v1 = a.dget((r1 - LR) / NRB); }
-            if (p < LR) { ownp = (rb == p % NRB); if (ownp) vp = Ls[p][c]; }
-            else { ownp = (rb == (p - LR) % NRB); if (ownp) vp = a.dget((p - LR) / NRB); }
-            int Lnew = L;
-            if (sw) { if (L == r1) Lnew = p; else if (L == p) Lnew = r1; }
-            if (ownk) XYs[Lnew].x = x;
-            if (ownp) XYs[Lnew].y = vp;
-            if (sw && own1) EXs[c] = v1;
-            __syncthreads();   // B2
-            if (sw) {
-                if (own1) {
-                    const T nv = XYs[Lnew].y;
-                    if (r1 < LR) Ls[r1][c] = nv; else a.dset((r1 - LR) / NRB, nv);
+            // 3. row q -> (X, Y)
+            bool ownq;
+            T y = zero<T>();
+            if (q < LR) { ownq = (rb == q % NRB); if (ownq) y = Ls[q][c]; }
+            else { ownq = (rb == (q - LR) % NRB); if (ownq) y = a.dget((q - LR) / NRB); }
+            if (ownq) {
+                XY v{Xs[c], y};
+                if constexpr (MODE == MODE_GRAD) {
+                    if (c == f) { v.x = -one<T>(); v.y = -piv; }
+                    else if (c == q) { v.x = piv; v.y = -one<T>(); }
                 }
-                if (ownp) {
-                    const T ov = EXs[c];
-                    if (p < LR) Ls[p][c] = ov; else a.dset((p - LR) / NRB, ov);
-                }
+                XYs[c] = v;
             }
-            L = Lnew;
-            const T yc = XYs[L].y;
+            __syncthreads();   // B2
+            if (t == 0 && sw) {
+                pos[k + 1] = q; pos[lq] = p1;
+                lab[q] = k + 1; lab[p1] = lq;
+            }
 
-            if (MODE == MODE_PROD) { prod = prod * piv; if (sw) prod = -prod; }
+            if constexpr (MODE == MODE_PROD) { prod = prod * piv; if (sw) prod = -prod; }
             else { la.mul(absd(piv)); ph.mul(piv); if (sw) ph.flip(); }
 
-            // 4. rank-2 update of rows >= k+2
+            // 4. rank-2 update of every row
             const T inv = recip(piv);
-            const T xs = xc * inv, ys = yc * inv;
-            // LDS row r always belongs to thread block r % NRB, the same thread
-            // that wrote it in the interchange above, so no barrier is needed.
-            if (LR > 0 && k + 2 < LR) {
-                const int r_first = k + 2 + ((rb - (k + 2)) % NRB + NRB) % NRB;
-                for (int r = r_first; r < LR && r < n; r += NRB) {
-                    const XY q = XYs[r];
-                    Ls[r][c] = mul_add_update(Ls[r][c], q.y, xs, q.x, ys);
+            const XY own = XYs[c];
+            T xs = own.x * inv, ys = own.y * inv;
+            if constexpr (MODE == MODE_GRAD) {
+                if (c == f) { xs = -inv; ys = -one<T>(); }
+                else if (c == q) { xs = one<T>(); ys = -inv; }
+            }
+            if (LR > 0) {
+                for (int r = rb; r < LR && r < n; r += NRB) {
+                    const XY qv = XYs[r];
+                    Ls[r][c] = mul_add_update(Ls[r][c], qv.y, xs, qv.x, ys);
                 }
             }
-            // Register rows in groups of G with one uniform guard per group:
-            // rows already eliminated or past n inside a live group take a
-            // harmless update (nothing reads them again), and the group's LDS
-            // reads issue back to back.
-            constexpr int G = 8;
+            constexpr int G = (RB * (int)sizeof(T) >= 384) ? 2 : 4;   // rows whose LDS reads issue together
 #pragma unroll
             for (int g = 0; g < RB; g += G) {
-                const int lo = LR + g * NRB;                          // lowest row of the group
-                const int hi = LR + (g + G - 1) * NRB + NRB - 1;      // highest row of the group
-                if (hi >= k + 2 && lo < n) {
+                if (LR + g * NRB < n) {
 #pragma unroll
                     for (int i = g; i < g + G && i < RB; ++i) {
-                        const XY q = XYs[LR + i * NRB + rb];
-                        a.set(i, mul_add_update(a.get(i), q.y, xs, q.x, ys));
+                        const XY qv = XYs[LR + i * NRB + rb];
+                        a.set(i, mul_add_update(a.get(i), qv.y, xs, qv.x, ys));
                     }
                 }
             }
+            f = f_next;
         }
 
         if (t == 0) {
-            if (MODE == MODE_PROD) {
+            if constexpr (MODE == MODE_PROD) {
                 ((T*)d.prod_out)[idx] = dead ? zero<T>() : prod;
             } else {
                 d.log_out[idx] = dead ? -INFINITY : la.value();
@@ -201,6 +214,22 @@ pf_reg_kernel(BatchDesc d) {
                 if constexpr (is_cplx<T>::value) { phv.re = (R)ph.re; phv.im = (R)ph.im; }
                 else phv = (T)ph.s;
                 ((T*)d.phase_out)[idx] = dead ? zero<T>() : phv;
+            }
+        }
+        if constexpr (MODE == MODE_GRAD) {
+            // physical order is the caller's order: G = -1/2 A^-1 = 1/2 A^-T
+            T* G = (T*)d.grad_out + off;
+            const R h = (R)(-0.5);
+            if (c < n) {
+#pragma unroll
+                for (int i = 0; i < RB; ++i) {
+                    const int r = LR + i * NRB + rb;
+                    if (r < n) G[(int64_t)r * n + c] = dead ? nan_of<T>() : a.get(i) * h;
+                }
+                if (LR > 0) {
+                    for (int r = rb; r < LR && r < n; r += NRB)
+                        G[(int64_t)r * n + c] = dead ? nan_of<T>() : Ls[r][c] * h;
+                }
             }
         }
     }
@@ -293,9 +322,49 @@ template <> int launch_typed<c64, MODE_PROD>(const BatchDesc& d, int maxn, hipSt
     PFH_CFG(32, 2, 0) PFH_CFG(64, 4, 0)
     return (int)hipErrorInvalidValue;
 }
+// Gradient (block sweep): register rows only (no LDS rows).
+template <> int launch_typed<double, MODE_GRAD>(const BatchDesc& d, int maxn, hipStream_t s) {
+    using T = double; constexpr int MODE = MODE_GRAD;
+    PFH_CFG(32, 2, 0) PFH_CFG(64, 4, 0) PFH_CFG(128, 4, 0) PFH_CFG(192, 4, 0)
+    return (int)hipErrorInvalidValue;
+}
+template <> int launch_typed<c128, MODE_GRAD>(const BatchDesc& d, int maxn, hipStream_t s) {
+    using T = c128; constexpr int MODE = MODE_GRAD;
+    PFH_CFG(32, 2, 0) PFH_CFG(64, 4, 0) PFH_CFG(128, 8, 0)
+    return (int)hipErrorInvalidValue;
+}
+template <> int launch_typed<float, MODE_GRAD>(const BatchDesc& d, int maxn, hipStream_t s) {
+    using T = float; constexpr int MODE = MODE_GRAD;
+    PFH_CFG(32, 2, 0) PFH_CFG(64, 4, 0) PFH_CFG(128, 4, 0)
+    return (int)hipErrorInvalidValue;
+}
+template <> int launch_typed<c64, MODE_GRAD>(const BatchDesc& d, int maxn, hipStream_t s) {
+    using T = c64; constexpr int MODE = MODE_GRAD;
+    PFH_CFG(32, 2, 0) PFH_CFG(64, 4, 0) PFH_CFG(128, 4, 0)
+    return (int)hipErrorInvalidValue;
+}
 #undef PFH_CFG
 
+int reg_grad_max_n(int dtype) {
+    switch (dtype) {
+        case DT_F64: return 192;
+        case DT_F32: return 128;
+        case DT_C64: return 128;
+        case DT_C128: return 128;
+    }
+    return 0;
+}
+
 int launch_reg(int dtype, int mode, const BatchDesc& d, int maxn, hipStream_t s) {
+    if (mode == MODE_GRAD) {
+        switch (dtype) {
+            case DT_F32: return launch_typed<float, MODE_GRAD>(d, maxn, s);
+            case DT_F64: return launch_typed<double, MODE_GRAD>(d, maxn, s);
+            case DT_C64: return launch_typed<c64, MODE_GRAD>(d, maxn, s);
+            case DT_C128: return launch_typed<c128, MODE_GRAD>(d, maxn, s);
+        }
+        return (int)hipErrorInvalidValue;
+    }
     if (mode == MODE_PROD) {
         switch (dtype) {
             case DT_F32: return launch_typed<float, MODE_PROD>(d, maxn, s);

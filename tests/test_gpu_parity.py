@@ -181,3 +181,39 @@ def test_blocked_zero_and_identity(pf, cuda):
         M, want = oracle.tridiag_identity_case(rng, n)
         la, s = pf.slog_pfaffian(torch.from_numpy(M).to(cuda), method=pf.PFH_METHOD_BLOCKED)
         assert np.isclose(want, float(s) * math.exp(float(la)), rtol=1e-10, atol=1e-12), n
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
+@pytest.mark.parametrize("n", [8, 32, 64, 128, 200, 300])
+def test_grad_is_half_inverse_transpose(pf, cuda, dtype, n):
+    from pfcuda_amd.adjoint import slog_grad
+    if dtype.is_complex and n == 200:
+        pytest.skip("covered by 300 (global sweep) and 128 (register sweep)")
+    rng = np.random.default_rng(100 + n)
+    A = oracle.skew_gaussian(rng, n, complex_=dtype.is_complex, batch=2)
+    G, la, ph = slog_grad(torch.from_numpy(A).to(cuda), return_slog=True)
+    G = G.cpu().numpy()
+    for b in range(2):
+        want = 0.5 * np.linalg.inv(A[b]).T
+        err = np.abs(G[b] - want).max() / np.abs(want).max()
+        assert err < 1e-10, (n, err)
+        l0, p0 = oracle.slog(A[b])
+        assert abs(float(la[b]) - l0) < 1e-9 and abs(complex(ph[b].item()) - p0) < 1e-9
+
+
+def test_grad_autograd_and_finite_differences(pf, cuda):
+    # C5 check: central finite differences (h = 1e-6) along skew directions
+    rng = np.random.default_rng(55)
+    n = 40
+    A0 = oracle.skew_gaussian(rng, n)
+    A = torch.from_numpy(A0).to(cuda).requires_grad_(True)
+    la, _ = pf.slog_pfaffian(A)
+    (G,) = torch.autograd.grad(la, A)
+    G = G.cpu().numpy()
+    h = 1e-6
+    for (i, j) in [(0, 1), (3, 17), (10, 39), (25, 26)]:
+        E = np.zeros((n, n)); E[i, j] = 1.0; E[j, i] = -1.0
+        fp = oracle.slog(A0 + h * E)[0]
+        fm = oracle.slog(A0 - h * E)[0]
+        fd = (fp - fm) / (2 * h)
+        assert abs((G[i, j] - G[j, i]) - fd) < 1e-6 * max(1.0, abs(fd)), (i, j, G[i, j] - G[j, i], fd)
