@@ -42,6 +42,9 @@
 // become -y/a and x/a).  At the end the matrix is A^-1 in the caller's
 // order, and the gradient of log|Pf| (the reference's JVP,
 // pfcuda/cuda_api.py:93-107, transposed) is 1/2 A^-T = -1/2 A^-1.
+#include <cstdlib>
+#include <cstring>
+
 #include "pf_batch.h"
 #include "pf_regstore.h"
 
@@ -355,7 +358,31 @@ int reg_grad_max_n(int dtype) {
     return 0;
 }
 
+// FP64 slog / product: the MFMA-tile kernel (pf_tile.hip) unless
+// PFH_F64_KERNEL=lanes selects the column-per-lane kernel above.
+// f64 kernel choice: "lanes" (column per lane) or "tile" (MFMA tiles).  By
+// default the tile kernel takes slog buckets above 128, where its rank-4 MFMA
+// update beats the lanes kernel's full-column FMA sweep; PFH_F64_KERNEL forces one.
+static int f64_kernel_override() {
+    static int v = -2;
+    if (v == -2) {
+        const char* e = getenv("PFH_F64_KERNEL");
+        v = !e ? -1 : strcmp(e, "lanes") == 0 ? 0 : strcmp(e, "tile") == 0 ? 1 : -1;
+    }
+    return v;
+}
+
+static bool use_tile_f64(int mode, int maxn) {
+    const int o = f64_kernel_override();
+    if (mode == MODE_PROD && maxn > 64) return false;
+    if (mode == MODE_GRAD) return false;
+    if (o >= 0) return o == 1;
+    return mode == MODE_SLOG && maxn > 128;
+}
+
 int launch_reg(int dtype, int mode, const BatchDesc& d, int maxn, hipStream_t s) {
+    if (dtype == DT_F64 && use_tile_f64(mode, maxn))
+        return launch_tile_f64(mode, d, maxn, s);
     if (mode == MODE_GRAD) {
         switch (dtype) {
             case DT_F32: return launch_typed<float, MODE_GRAD>(d, maxn, s);
