@@ -7,10 +7,15 @@ and resident in HBM before timing.  One step = one pfh_slog_pfaffian_ragged
 call over the whole batch (device-side size bucketing + one persistent launch
 per bucket).  Prints one JSON line.
 
-Also reported: the small batched product path (1024 x 32x32 f64), the
-dominant kernel's roofline fraction (HIP events on the launch stream), the
-accuracy of a sample against the FP64 CPU oracle, and the CPU oracle's own
-throughput on that sample (cpu_baseline).
+Also reported: the small batched product path (1024 x 32x32 f64); the
+dominant kernel's roofline fraction (HIP events on the launch stream) with
+its HBM bytes per launch from the committed rocprofv3 counter summary;
+single large matrices (N = 4096 f64 and c128, plain vs blocked; the N = 8192
+precision matrix Q diag(s_i J) Q^T against its closed form); the gradient
+1/2 A^-T (512 x 128^2 and 2048^2); the accuracy of a sample against the FP64
+CPU oracle and the oracle's own throughput over the whole batch
+(cpu_baseline, about 10 s of host time).  --profile runs only the ragged
+steps plus a 1 GiB device copy, the workload of the committed profiles.
 """
 from __future__ import annotations
 
@@ -84,6 +89,21 @@ class Ragged:
             self.stream)
         if st != 0:
             raise RuntimeError(f"pfh_slog_pfaffian_ragged status {st}")
+
+
+def committed_traffic(kernel_key):
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated against a copy of
+    known size); None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_ragged16384_f64.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch " + kernel_key + ":"):
+                    return float(line.split(":")[-1])
+    except OSError:
+        pass
+    return None
 
 
 def time_events(torch, fn, reps):
@@ -304,8 +324,9 @@ def main():
         "config": {"workload": "ragged slog_pfaffian: 16384 f64 skew matrices, even N uniform in [16,256]",
                    "count": args.count, "seed": args.seed, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK, "traffic": None,
-                     "kernel": "pf_reg_kernel<double,256,4,64,slog> (bucket 192<N<=256)",
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK,
+                     "traffic": committed_traffic("pf_reg_kernel<double,256,4,64,1>"),
+                     "kernel": "pf_reg_kernel<double,256,4,64,1> (bucket 192<N<=256)",
                      "launch_ms": t_big * 1e3, "algorithmic_flop": big_flops},
         "whole_batch": {"flop": flops, "tflops": flops / (ms / 1e3) / 1e12,
                         "frac_fp64_peak": flops / (ms / 1e3) / FP64_PEAK, "triangle_bytes": tri_bytes},

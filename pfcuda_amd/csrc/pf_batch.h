@@ -33,8 +33,9 @@ enum { DT_F32 = 0, DT_F64 = 1, DT_C64 = 2, DT_C128 = 3 };
 int reg_max_n(int dtype);
 int reg_grad_max_n(int dtype);
 
-// FP64 MFMA-tile kernels (pf_tile.hip): MODE_SLOG up to n = 256, MODE_PROD up to 64.
-int launch_tile_f64(int mode, const BatchDesc& d, int maxn, hipStream_t stream);
+// FP64 LDS-resident tile kernels (pf_lds.hip): MODE_SLOG up to n = 192, MODE_PROD up to 64.
+int launch_lds_f64(int mode, const BatchDesc& d, int maxn, hipStream_t stream);
+int lds_max_n();
 
 // Launches the workgroup kernel whose capacity covers `maxn` for every
 // matrix in the batch.  Returns 0 or a hipError_t value.

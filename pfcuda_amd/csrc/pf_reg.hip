@@ -358,31 +358,35 @@ int reg_grad_max_n(int dtype) {
     return 0;
 }
 
-// FP64 slog / product: the MFMA-tile kernel (pf_tile.hip) unless
-// PFH_F64_KERNEL=lanes selects the column-per-lane kernel above.
-// f64 kernel choice: "lanes" (column per lane) or "tile" (MFMA tiles).  By
-// default the tile kernel takes slog buckets above 128, where its rank-4 MFMA
-// update beats the lanes kernel's full-column FMA sweep; PFH_F64_KERNEL forces one.
+// f64 kernel choice for slog and the product form: "lanes" (column per lane,
+// above) or "lds" (matrix in LDS, MFMA tile updates, pf_lds.hip).  Default,
+// from per-bucket timings of the 16384-matrix ragged batch on MI355X: slog
+// takes lds for 64 < n <= 192 and lanes elsewhere; the product form keeps
+// lanes, whose rank-2 updates round as the reference's (rel 1e-12 on
+// 32 x 32).  PFH_F64_KERNEL=lanes|lds forces one where it applies.
 static int f64_kernel_override() {
     static int v = -2;
     if (v == -2) {
         const char* e = getenv("PFH_F64_KERNEL");
-        v = !e ? -1 : strcmp(e, "lanes") == 0 ? 0 : strcmp(e, "tile") == 0 ? 1 : -1;
+        v = !e ? -1 : strcmp(e, "lanes") == 0 ? 0 : strcmp(e, "lds") == 0 ? 2 : -1;
     }
     return v;
 }
 
-static bool use_tile_f64(int mode, int maxn) {
+// 0 lanes, 2 lds
+static int f64_kernel(int mode, int maxn) {
+    if (mode == MODE_GRAD) return 0;
+    if (maxn > lds_max_n() || (mode == MODE_PROD && maxn > 64)) return 0;
     const int o = f64_kernel_override();
-    if (mode == MODE_PROD && maxn > 64) return false;
-    if (mode == MODE_GRAD) return false;
-    if (o >= 0) return o == 1;
-    return mode == MODE_SLOG && maxn > 128;
+    if (o >= 0) return o;
+    if (mode == MODE_PROD) return 0;
+    return maxn > 64 ? 2 : 0;
 }
 
 int launch_reg(int dtype, int mode, const BatchDesc& d, int maxn, hipStream_t s) {
-    if (dtype == DT_F64 && use_tile_f64(mode, maxn))
-        return launch_tile_f64(mode, d, maxn, s);
+    if (dtype == DT_F64) {
+        if (f64_kernel(mode, maxn) == 2) return launch_lds_f64(mode, d, maxn, s);
+    }
     if (mode == MODE_GRAD) {
         switch (dtype) {
             case DT_F32: return launch_typed<float, MODE_GRAD>(d, maxn, s);

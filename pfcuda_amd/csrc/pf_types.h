@@ -109,7 +109,11 @@ struct LogAcc {
         m = frexp(m, &ex);
         e += ex;
     }
-    __device__ double value() const { return log(m) + (double)e * 0.69314718055994530942; }
+    // out of line: inlined, the log's constants get hoisted into registers
+    // that stay live across the whole elimination
+    __device__ __attribute__((noinline)) double value() const {
+        return log(m) + (double)e * 0.69314718055994530942;
+    }
 };
 
 }  // namespace pfh
