@@ -36,7 +36,7 @@ extern "C" {
 #define PFH_METHOD_AUTO 0
 #define PFH_METHOD_UNBLOCKED 1   /* plain all-FP64 path, one rank-2 update per pair step */
 #define PFH_METHOD_BLOCKED 2     /* blocked, FP64 trailing update */
-#define PFH_METHOD_MIXED 3       /* blocked, split-precision trailing update */
+#define PFH_METHOD_MIXED 3       /* reserved for a split-precision trailing update; runs BLOCKED (FP64) today */
 
 /* Short text for a status code. */
 const char* pfh_status_string(int status);
