@@ -23,6 +23,8 @@ struct BatchDesc {
     double* log_out;          // MODE_SLOG: log|Pf|
     void* phase_out;          // MODE_SLOG: sign / unit phase as the input type
     void* grad_out;           // MODE_GRAD: 1/2 A^-T, laid out like the input
+    double* slab;             // f64 slog, 192 < n <= 256: one spill slab per resident workgroup
+    int64_t slab_elems;       //   (lds_spill_elems() doubles each), nullptr when absent
 };
 
 enum { MODE_PROD = 0, MODE_SLOG = 1, MODE_GRAD = 2 };
@@ -33,9 +35,13 @@ enum { DT_F32 = 0, DT_F64 = 1, DT_C64 = 2, DT_C128 = 3 };
 int reg_max_n(int dtype);
 int reg_grad_max_n(int dtype);
 
-// FP64 LDS-resident tile kernels (pf_lds.hip): MODE_SLOG up to n = 192, MODE_PROD up to 64.
+// FP64 LDS-resident kernels (pf_lds.hip): MODE_SLOG up to n = 256 (above
+// 192 with d.slab set: the first rows of the triangle spill to a global
+// slab per workgroup), MODE_PROD up to 64.
 int launch_lds_f64(int mode, const BatchDesc& d, int maxn, hipStream_t stream);
-int lds_max_n();
+int lds_max_n();                 // 192 without spill slabs
+int64_t lds_spill_elems();       // doubles per spill slab
+int lds_spill_slabs();           // slabs needed (one per resident workgroup)
 
 // Launches the workgroup kernel whose capacity covers `maxn` for every
 // matrix in the batch.  Returns 0 or a hipError_t value.

@@ -123,7 +123,7 @@ int bucket_edges(int dt, int32_t* e) {
 }
 
 struct RaggedLayout {
-    size_t hist, cursor, order, begin, cnt, edges, slabs, total;
+    size_t hist, cursor, order, begin, cnt, edges, spill, slabs, total;
     int glb_grid;
     int64_t slab_elems;
 };
@@ -144,6 +144,9 @@ RaggedLayout ragged_layout(int dt, int64_t count, int32_t max_n) {
     L.begin = off; off = align_up(off + sizeof(int32_t) * (kMaxBuckets + 1));
     L.cnt = off; off = align_up(off + sizeof(int32_t) * (kMaxBuckets + 1));
     L.edges = off; off = align_up(off + sizeof(int32_t) * (kMaxBuckets + 1));
+    L.spill = off;
+    if (dt == PFH_F64 && max_n > lds_max_n())   // spill slabs of the LDS kernel's 192 < n <= 256 bucket
+        off = align_up(off + sizeof(double) * (size_t)lds_spill_elems() * lds_spill_slabs());
     L.slabs = off;
     L.glb_grid = 0;
     L.slab_elems = 0;
@@ -196,6 +199,10 @@ int ragged_typed(int dt, const void* data, const int64_t* offsets, const int32_t
         d.order = order; d.begin_dev = bbegin + b; d.count_dev = bcount + b;
         d.count = count; d.dense_n = 0;
         d.log_out = log_out; d.phase_out = phase_out;
+        if (dt == PFH_F64 && max_n > lds_max_n()) {
+            d.slab = (double*)(ws + L.spill);
+            d.slab_elems = lds_spill_elems();
+        }
         if (b >= nb) {
             err = launch_glb<T>(d, (T*)(ws + L.slabs), L.slab_elems, L.glb_grid, s);
         } else {
@@ -222,8 +229,13 @@ int pfaffian_dense(int dt, const void* A, int64_t batch, int32_t n, void* out, h
     return launch_reg(dt, MODE_PROD, d, n < 2 ? 2 : n, s);
 }
 
+size_t dense_spill_bytes(int dt, int32_t n) {
+    if (dt != PFH_F64 || n <= lds_max_n() || n > reg_max_n(dt)) return 0;
+    return sizeof(double) * (size_t)lds_spill_elems() * lds_spill_slabs();
+}
+
 size_t slog_ws_bytes(int dt, int64_t batch, int32_t n, int method) {
-    if (n <= reg_max_n(dt) && method == PFH_METHOD_AUTO) return 0;
+    if (n <= reg_max_n(dt) && method == PFH_METHOD_AUTO) return dense_spill_bytes(dt, n);
     const size_t es = dtype_bytes(dt);
     if (method == PFH_METHOD_BLOCKED || method == PFH_METHOD_MIXED ||
         (method == PFH_METHOD_AUTO && n > reg_max_n(dt)))
@@ -241,6 +253,14 @@ int slog_dense(int dt, const T* A, int64_t batch, int32_t n, int method, double*
     if (method == PFH_METHOD_AUTO && n <= reg_max_n(dt)) {
         BatchDesc d{};
         d.data = A; d.count = batch; d.dense_n = n; d.log_out = log_out; d.phase_out = phase_out;
+        const size_t sb = dense_spill_bytes(dt, n);
+        if (sb) {
+            void* ws = nullptr;
+            int err = resolve_ws(ws_in, ws_bytes, sb, &ws);
+            if (err) return err;
+            d.slab = (double*)ws;
+            d.slab_elems = lds_spill_elems();
+        }
         return launch_reg(dt, MODE_SLOG, d, n < 2 ? 2 : n, s);
     }
     if (n == 0 || (n & 1)) {

@@ -5,7 +5,10 @@
 // off(i) = i*(N16-2) - i*(i-1)/2 - 1.  A row r is then one LDS read per
 // position c: off(r) + c for c > r, and -(off(c) + r) for c < r.  At
 // n <= 192 the triangle (146688 bytes) and the four update vectors fit one
-// CU's 160 KB of LDS.
+// CU's 160 KB of LDS.  Above 192 the first S = N16 - 192 rows (a prefix of
+// the packed order, and the first to be eliminated) live in a global slab
+// owned by the workgroup and the remaining 192 rows in LDS: after S/2 pair
+// steps the matrix is LDS-only.
 //
 // Pivoting is physical, as in the reference (pf_sm_calc swaps rows and
 // columns k+1 and kp, src/pfaffian_sm.cu:48-66), so the trailing matrix
@@ -126,12 +129,14 @@ template <int NP, int W, int MODE>
 __global__ void __launch_bounds__(W * 64)
 pf_lds_kernel(BatchDesc d) {
     constexpr int NM = (NP + 63) / 64;          // positions per lane in the pivot wave
-    constexpr int TRI = NP * (NP - 1) / 2;      // triangle capacity
+    constexpr bool SPILL = NP > 192;            // first rows in a global slab
+    constexpr int TRI = SPILL ? 192 * 191 / 2 : NP * (NP - 1) / 2;   // LDS triangle capacity
+    constexpr int SINK = TRI + 4 * NP;          // 256 doubles: sink for masked tile writes
     extern __shared__ double lds_raw[];
-    lds_f64* P = (lds_f64*)lds_raw;             // packed triangle
+    lds_f64* P = (lds_f64*)lds_raw;             // packed triangle (rows >= S)
     lds_f64* OPV = P + TRI;                     // [U_A, X_A, U_B, X_B], NP each
-    lds_f64* DUMMY = OPV + 4 * NP;              // 256 doubles: sink for masked tile writes
-    lds_i32* flag = (lds_i32*)(DUMMY + 256);
+    lds_i32* flag = (lds_i32*)(P + SINK + 256);
+    double* G = SPILL ? d.slab + (int64_t)blockIdx.x * d.slab_elems : nullptr;   // rows < S
 
     const int t = threadIdx.x;
     const int lane = t & 63;
@@ -167,12 +172,23 @@ pf_lds_kernel(BatchDesc d) {
         }
         const int NTn = (n + 15) >> 4;
         const int N16 = 16 * NTn;
+        // rows < S live in G at their packed index, the rest in P at index - G0
+        const int S = (SPILL && N16 > 192) ? N16 - 192 : 0;
+        const int G0 = S * (N16 - 1) - (S * (S - 1)) / 2;
+        auto ld = [&](int ix) -> double {
+            if (SPILL && ix < G0) return G[ix];
+            return P[ix - G0];
+        };
+        auto st = [&](int ix, double v) {
+            if (SPILL && ix < G0) G[ix] = v;
+            else P[ix - G0] = v;
+        };
 
         // ---- load the strict upper triangle, one row per wave at a time ----
         for (int i = w; i < N16; i += W) {
             const int oi = row_off(i, N16);
             for (int jj = i + 1 + lane; jj < N16; jj += 64)
-                P[oi + jj] = (i < n && jj < n) ? base[(int64_t)i * n + jj] : 0.0;
+                st(oi + jj, (i < n && jj < n) ? base[(int64_t)i * n + jj] : 0.0);
         }
 
         // per-lane constants
@@ -194,25 +210,26 @@ pf_lds_kernel(BatchDesc d) {
         __syncthreads();
 
         // row r at mapped positions p[m] (offsets po[m] = off(p[m])) of this lane
-        auto read_row = [&](int r, const int (&p)[NM], const int (&po)[NM], double (&v)[NM]) {
+        // (positions below lo are eliminated: not read, 0)
+        auto read_row = [&](int r, int lo, const int (&p)[NM], const int (&po)[NM], double (&v)[NM]) {
             const int sr = row_off(r, N16);
 #pragma unroll
             for (int m = 0; m < NM; ++m) {
                 const int pm = p[m];
                 const int ix = pm > r ? sr + pm : po[m] + r;
                 double x = 0.0;
-                if (cpos[m] < N16) x = P[ix];
-                v[m] = (pm == r) ? 0.0 : (pm > r ? x : -x);
+                if (cpos[m] < N16 && pm >= lo && pm != r) x = ld(ix);
+                v[m] = pm > r ? x : -x;
             }
         };
         // row and column r := v at mapped positions
-        auto write_row = [&](int r, const int (&p)[NM], const int (&po)[NM], const double (&v)[NM]) {
+        auto write_row = [&](int r, int lo, const int (&p)[NM], const int (&po)[NM], const double (&v)[NM]) {
             const int sr = row_off(r, N16);
 #pragma unroll
             for (int m = 0; m < NM; ++m) {
                 const int pm = p[m];
                 const int ix = pm > r ? sr + pm : po[m] + r;
-                if (cpos[m] < N16 && pm != r) P[ix] = pm > r ? v[m] : -v[m];
+                if (cpos[m] < N16 && pm >= lo && pm != r) st(ix, pm > r ? v[m] : -v[m]);
             }
         };
 
@@ -227,9 +244,10 @@ pf_lds_kernel(BatchDesc d) {
                 // Vectors are kept in the panel's starting coordinates; an
                 // interchange only changes where an LDS access lands.
                 double x[NM], z[NM], r2[NM], r3[NM];
-                read_row(k, cpos, coff, x);
-                read_row(k + 1, cpos, coff, z);
-                if (has_b) { read_row(k + 2, cpos, coff, r2); read_row(k + 3, cpos, coff, r3); }
+                const int lo = (k >> 4) << 4;
+                read_row(k, lo, cpos, coff, x);
+                read_row(k + 1, lo, cpos, coff, z);
+                if (has_b) { read_row(k + 2, lo, cpos, coff, r2); read_row(k + 3, lo, cpos, coff, r3); }
                 int st = 0;
                 // ---- pair A ----
                 const int pA = k + 1;
@@ -239,7 +257,7 @@ pf_lds_kernel(BatchDesc d) {
                 if (!st) {
                     const bool swA = (qA != pA);
                     double ua[NM];
-                    read_row(qA, cpos, coff, ua);
+                    read_row(qA, lo, cpos, coff, ua);
                     const double inva = 1.0 / a;
 #pragma unroll
                     for (int m = 0; m < NM; ++m) ua[m] *= inva;
@@ -252,7 +270,7 @@ pf_lds_kernel(BatchDesc d) {
                         pa[m] = swap_pos(c, pA, qA);
                         pao[m] = c == pA ? offqA : (c == qA ? offpA : coff[m]);
                     }
-                    if (swA) write_row(qA, pa, pao, z);
+                    if (swA) write_row(qA, lo, pa, pao, z);
                     if constexpr (MODE == MODE_PROD) { prod *= a; if (swA) prod = -prod; }
                     else { la.mul(fabs(a)); ph.mul(a); if (swA) ph.flip(); }
 
@@ -272,7 +290,7 @@ pf_lds_kernel(BatchDesc d) {
                         if (!st) {
                             const bool swB = (qB != pB);
                             double ub[NM];
-                            read_row(qB, pa, pao, ub);   // row qB after interchange A (written above)
+                            read_row(qB, lo, pa, pao, ub);   // row qB after interchange A (written above)
                             const int oqB = swap_pos(qB, pA, qA);
                             const double uaq = pos_value(ua, oqB), xaq = pos_value(x, oqB);
                             const double invb = 1.0 / b;
@@ -290,7 +308,7 @@ pf_lds_kernel(BatchDesc d) {
                                 double zb[NM];
 #pragma unroll
                                 for (int m = 0; m < NM; ++m) zb[m] = pbz ? z[m] : r3[m];
-                                write_row(qB, pab, pabo, zb);
+                                write_row(qB, lo, pab, pabo, zb);
                             }
                             if constexpr (MODE == MODE_PROD) { prod *= b; if (swB) prod = -prod; }
                             else { la.mul(fabs(b)); ph.mul(b); if (swB) ph.flip(); }
@@ -325,48 +343,52 @@ pf_lds_kernel(BatchDesc d) {
                 const double qs = (lq & 1) ? -1.0 : 1.0;
                 int I = I0, J = I0 + w;
                 while (J >= NTn && I < NTn) { ++I; J = I + (J - NTn); }
-                constexpr int G = 4;
-                for (int tt = w; tt < nt; tt += W * G) {
-                    int ix[G][4];
-                    int oI[G], oJ[G];
-                    d4 acc[G];
+                constexpr int GB = 4;
+                for (int tt = w; tt < nt; tt += W * GB) {
+                    int ix[GB][4];
+                    int oI[GB], oJ[GB];
+                    bool gI[GB];
+                    d4 acc[GB];
 #pragma unroll
-                    for (int g = 0; g < G; ++g) {
+                    for (int g = 0; g < GB; ++g) {
                         const bool ok = (tt + W * g < nt);
                         const bool dg = ok && (I == J);
                         const int sI = 16 * I * (N16 - 2) - 128 * I * I + 8 * I + 16 * J;   // S(I) + 16 J
                         const int tI = 16 * I;
+                        const bool inG = SPILL && ok && (16 * I < S);   // tile rows in the global slab
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
-                            const int ia = toff[r] - tI * tx[r] + sI;   // (16I + tx, 16J + lr), above the diagonal
-                            const int ib = toffl[r] - tI * lr + sI;     // diagonal tile, below its diagonal: (j, i)
-                            int e = dg && !tlow[r] ? ib : ia;
-                            e = ok ? e : (TRI + 4 * NP + r * 64 + lane);
-                            e = (dg && tx[r] == lr) ? (TRI + 4 * NP + r * 64 + lane) : e;
-                            ix[g][r] = e;
-                            const double v = P[e];
+                            const int ia = toff[r] - __mul24(tI, tx[r]) + sI;   // (16I + tx, 16J + lr), above the diagonal
+                            const int ib = toffl[r] - __mul24(tI, lr) + sI;     // diagonal tile, below its diagonal: (j, i)
+                            const int e = dg && !tlow[r] ? ib : ia;
+                            const bool sink = !ok || (dg && tx[r] == lr);
+                            double v;
+                            if (inG) v = sink ? 0.0 : G[e];
+                            else v = P[sink ? SINK + r * 64 + lane : e - G0];
                             acc[g][r] = dg ? (tx[r] == lr ? 0.0 : (tlow[r] ? v : -v)) : v;
+                            // entries on or below a diagonal tile's diagonal are not stored
+                            ix[g][r] = (!ok || (dg && !tlow[r])) ? -1 : e;
                         }
+                        gI[g] = inG;
                         oI[g] = ok ? 16 * I : 0;
                         oJ[g] = ok ? 16 * J : 0;
-                        // sink writes of entries on or below a diagonal tile's diagonal
-                        if (dg) {
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) ix[g][r] = tlow[r] ? ix[g][r] : (TRI + 4 * NP + r * 64 + lane);
-                        }
                         J += W;
                         while (J >= NTn && I < NTn) { ++I; J = I + (J - NTn); }
                     }
 #pragma unroll
-                    for (int g = 0; g < G; ++g) {
+                    for (int g = 0; g < GB; ++g) {
                         const double av = Pv[oI[g] + lr];
                         const double bv = qs * Qv[oJ[g] + lr];
                         acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[g], 0, 0, 0);
                     }
 #pragma unroll
-                    for (int g = 0; g < G; ++g) {
+                    for (int g = 0; g < GB; ++g) {
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) P[ix[g][r]] = acc[g][r];
+                        for (int r = 0; r < 4; ++r) {
+                            const int e = ix[g][r];
+                            if (gI[g]) { if (e >= 0) G[e] = acc[g][r]; }
+                            else P[e >= 0 ? e - G0 : SINK + r * 64 + lane] = acc[g][r];
+                        }
                     }
                 }
             }
@@ -397,7 +419,9 @@ int lds_num_cu() {
 }
 
 template <int NP>
-constexpr size_t lds_bytes() { return sizeof(double) * (NP * (NP - 1) / 2 + 4 * NP + 256 + 1); }
+constexpr size_t lds_bytes() {
+    return sizeof(double) * ((NP > 192 ? 192 * 191 / 2 : NP * (NP - 1) / 2) + 4 * NP + 256 + 1);
+}
 
 template <int NP, int W, int MODE>
 int lds_launch(const BatchDesc& d, hipStream_t s) {
@@ -413,6 +437,11 @@ int lds_launch(const BatchDesc& d, hipStream_t s) {
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, W * 64, bytes) != hipSuccess || occ <= 0) occ = 1;
     int64_t grid = (int64_t)occ * lds_num_cu();
+    if (NP > 192) {
+        // one spill slab per workgroup
+        if (!d.slab || d.slab_elems < lds_spill_elems()) return (int)hipErrorInvalidValue;
+        if (grid > lds_spill_slabs()) grid = lds_spill_slabs();
+    }
     if (grid > d.count) grid = d.count;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(W * 64), bytes, s, d);
     return (int)hipGetLastError();
@@ -431,6 +460,11 @@ int lds_waves_override() {
 
 int lds_max_n() { return 192; }
 
+// rows 0..63 of a 256-padded matrix: 64 * 255 - 64 * 63 / 2 doubles
+int64_t lds_spill_elems() { return 64 * 255 - (64 * 63) / 2; }
+
+int lds_spill_slabs() { return lds_num_cu(); }
+
 int launch_lds_f64(int mode, const BatchDesc& d, int maxn, hipStream_t s) {
     const int ow = lds_waves_override();
     if (mode == MODE_SLOG) {
@@ -446,6 +480,10 @@ int launch_lds_f64(int mode, const BatchDesc& d, int maxn, hipStream_t s) {
         if (maxn <= 192) {
             if (ow == 4) return lds_launch<192, 4, MODE_SLOG>(d, s);
             return lds_launch<192, 8, MODE_SLOG>(d, s);
+        }
+        if (maxn <= 256) {
+            if (ow == 4) return lds_launch<256, 4, MODE_SLOG>(d, s);
+            return lds_launch<256, 8, MODE_SLOG>(d, s);
         }
     } else if (mode == MODE_PROD) {
         if (maxn <= 32) return lds_launch<32, 1, MODE_PROD>(d, s);

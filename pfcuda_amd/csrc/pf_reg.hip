@@ -359,11 +359,12 @@ int reg_grad_max_n(int dtype) {
 }
 
 // f64 kernel choice for slog and the product form: "lanes" (column per lane,
-// above) or "lds" (matrix in LDS, MFMA tile updates, pf_lds.hip).  Default,
-// from per-bucket timings of the 16384-matrix ragged batch on MI355X: slog
-// takes lds for 64 < n <= 192 and lanes elsewhere; the product form keeps
-// lanes, whose rank-2 updates round as the reference's (rel 1e-12 on
-// 32 x 32).  PFH_F64_KERNEL=lanes|lds forces one where it applies.
+// above) or "lds" (matrix in LDS, MFMA tile updates, pf_lds.hip; above
+// n = 192 only when the caller supplied spill slabs).  Default, from
+// per-bucket timings of the 16384-matrix ragged batch on MI355X: slog takes
+// lds for n > 64 and lanes up to 64; the product form keeps lanes, whose
+// rank-2 updates round as the reference's (rel 1e-12 on 32 x 32).
+// PFH_F64_KERNEL=lanes|lds forces one where it applies.
 static int f64_kernel_override() {
     static int v = -2;
     if (v == -2) {
@@ -374,9 +375,10 @@ static int f64_kernel_override() {
 }
 
 // 0 lanes, 2 lds
-static int f64_kernel(int mode, int maxn) {
+static int f64_kernel(int mode, int maxn, const BatchDesc& d) {
     if (mode == MODE_GRAD) return 0;
-    if (maxn > lds_max_n() || (mode == MODE_PROD && maxn > 64)) return 0;
+    if (mode == MODE_PROD && maxn > 64) return 0;
+    if (maxn > 256 || (maxn > lds_max_n() && !d.slab)) return 0;
     const int o = f64_kernel_override();
     if (o >= 0) return o;
     if (mode == MODE_PROD) return 0;
@@ -385,7 +387,7 @@ static int f64_kernel(int mode, int maxn) {
 
 int launch_reg(int dtype, int mode, const BatchDesc& d, int maxn, hipStream_t s) {
     if (dtype == DT_F64) {
-        if (f64_kernel(mode, maxn) == 2) return launch_lds_f64(mode, d, maxn, s);
+        if (f64_kernel(mode, maxn, d) == 2) return launch_lds_f64(mode, d, maxn, s);
     }
     if (mode == MODE_GRAD) {
         switch (dtype) {
