@@ -334,6 +334,10 @@ pf_lds_kernel(BatchDesc d) {
             if (k + 4 >= n) break;
 
             // ---- rank-4 skew update of every live tile, four tiles per batch ----
+            // Per batch: every load (tile entries and MFMA operands) first,
+            // then the MFMAs, then the stores.  Off-diagonal tiles take the
+            // straight path; a diagonal tile (one per tile row) reads its
+            // lower half as the negated upper half and stores the upper half.
             {
                 const int I0 = (k + 4) >> 4;
                 const int mt = NTn - I0;
@@ -345,49 +349,72 @@ pf_lds_kernel(BatchDesc d) {
                 while (J >= NTn && I < NTn) { ++I; J = I + (J - NTn); }
                 constexpr int GB = 4;
                 for (int tt = w; tt < nt; tt += W * GB) {
-                    int ix[GB][4];
-                    int oI[GB], oJ[GB];
-                    bool gI[GB];
-                    d4 acc[GB];
+                    int Ig[GB], Jg[GB];
 #pragma unroll
                     for (int g = 0; g < GB; ++g) {
-                        const bool ok = (tt + W * g < nt);
-                        const bool dg = ok && (I == J);
-                        const int sI = 16 * I * (N16 - 2) - 128 * I * I + 8 * I + 16 * J;   // S(I) + 16 J
-                        const int tI = 16 * I;
-                        const bool inG = SPILL && ok && (16 * I < S);   // tile rows in the global slab
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int ia = toff[r] - __mul24(tI, tx[r]) + sI;   // (16I + tx, 16J + lr), above the diagonal
-                            const int ib = toffl[r] - __mul24(tI, lr) + sI;     // diagonal tile, below its diagonal: (j, i)
-                            const int e = dg && !tlow[r] ? ib : ia;
-                            const bool sink = !ok || (dg && tx[r] == lr);
-                            double v;
-                            if (inG) v = sink ? 0.0 : G[e];
-                            else v = P[sink ? SINK + r * 64 + lane : e - G0];
-                            acc[g][r] = dg ? (tx[r] == lr ? 0.0 : (tlow[r] ? v : -v)) : v;
-                            // entries on or below a diagonal tile's diagonal are not stored
-                            ix[g][r] = (!ok || (dg && !tlow[r])) ? -1 : e;
-                        }
-                        gI[g] = inG;
-                        oI[g] = ok ? 16 * I : 0;
-                        oJ[g] = ok ? 16 * J : 0;
+                        Ig[g] = (tt + W * g < nt) ? I : -1;
+                        Jg[g] = J;
                         J += W;
                         while (J >= NTn && I < NTn) { ++I; J = I + (J - NTn); }
                     }
+                    d4 acc[GB];
+                    double av[GB], bv[GB];
+                    int ix[GB][4];
 #pragma unroll
                     for (int g = 0; g < GB; ++g) {
-                        const double av = Pv[oI[g] + lr];
-                        const double bv = qs * Qv[oJ[g] + lr];
-                        acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[g], 0, 0, 0);
+                        const int I_ = Ig[g], J_ = Jg[g];
+                        if (I_ < 0) continue;
+                        const int sI = 16 * I_ * (N16 - 2) - 128 * I_ * I_ + 8 * I_ + 16 * J_;   // S(I) + 16 J
+                        const int tI = 16 * I_;
+                        const bool inG = SPILL && (16 * I_ < S);   // tile rows in the global slab
+                        if (I_ != J_) {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const int e = toff[r] - __mul24(tI, tx[r]) + sI;   // (16I + tx, 16J + lr)
+                                ix[g][r] = e;
+                                acc[g][r] = inG ? G[e] : P[e - G0];
+                            }
+                        } else {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const int ia = toff[r] - __mul24(tI, tx[r]) + sI;   // above the diagonal
+                                const int ib = toffl[r] - __mul24(tI, lr) + sI;     // below it: (j, i)
+                                const bool on = tx[r] == lr;
+                                const int e = tlow[r] ? ia : ib;
+                                double v = 0.0;
+                                if (!on) v = inG ? G[e] : P[e - G0];
+                                acc[g][r] = tlow[r] ? v : -v;
+                                ix[g][r] = tlow[r] ? e : -1;
+                            }
+                        }
+                        av[g] = Pv[16 * I_ + lr];
+                        bv[g] = qs * Qv[16 * J_ + lr];
                     }
 #pragma unroll
                     for (int g = 0; g < GB; ++g) {
+                        if (Ig[g] < 0) continue;
+                        acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g], bv[g], acc[g], 0, 0, 0);
+                    }
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int e = ix[g][r];
-                            if (gI[g]) { if (e >= 0) G[e] = acc[g][r]; }
-                            else P[e >= 0 ? e - G0 : SINK + r * 64 + lane] = acc[g][r];
+                    for (int g = 0; g < GB; ++g) {
+                        const int I_ = Ig[g];
+                        if (I_ < 0) continue;
+                        const bool inG = SPILL && (16 * I_ < S);
+                        if (I_ != Jg[g]) {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                if (inG) G[ix[g][r]] = acc[g][r];
+                                else P[ix[g][r] - G0] = acc[g][r];
+                            }
+                        } else {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const int e = ix[g][r];
+                                if (e >= 0) {
+                                    if (inG) G[e] = acc[g][r];
+                                    else P[e - G0] = acc[g][r];
+                                }
+                            }
                         }
                     }
                 }
