@@ -83,22 +83,49 @@ __global__ void rg_hist(const int32_t* sizes, int64_t count, int32_t max_n, int3
 
 // One thread: positions in descending-size order and bucket ranges.
 // Bucket b holds sizes in (edge[b-1], edge[b]] (edge[-1] = -1).
-__global__ void rg_scan(int32_t* hist, int32_t max_n, const int32_t* edges, int nb,
-                        int32_t* cursor, int32_t* bucket_begin, int32_t* bucket_count) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int32_t pos = 0;
-    for (int s = max_n; s >= 0; --s) {
+// Exclusive scan of the size histogram in descending size order (largest
+// first), then each bucket's [begin, begin + count) range of the order list.
+// 256 threads: each sums a contiguous run of bins, one thread scans the 256
+// partial sums in LDS, then every thread writes its run's cursors.
+__global__ void __launch_bounds__(256) rg_scan(const int32_t* hist, int32_t max_n, const int32_t* edges, int nb,
+                                               int32_t* cursor, int32_t* bucket_begin, int32_t* bucket_count) {
+    __shared__ int32_t part[256];
+    __shared__ int32_t total;
+    const int t = threadIdx.x;
+    const int nbins = max_n + 1;
+    const int per = (nbins + 255) / 256;
+    const int i0 = t * per, i1 = (i0 + per < nbins) ? i0 + per : nbins;   // descending positions i -> bin max_n - i
+    int32_t sum = 0;
+    for (int i = i0; i < i1; ++i) sum += hist[max_n - i];
+    part[t] = sum;
+    __syncthreads();
+    if (t == 0) {
+        int32_t run = 0;
+        for (int u = 0; u < 256; ++u) {
+            const int32_t v = part[u];
+            part[u] = run;
+            run += v;
+        }
+        total = run;
+    }
+    __syncthreads();
+    int32_t pos = part[t];
+    for (int i = i0; i < i1; ++i) {
+        const int s = max_n - i;
         cursor[s] = pos;
         pos += hist[s];
     }
-    for (int b = 0; b < nb; ++b) {
-        const int hi = edges[b] > max_n ? max_n : edges[b];
-        const int lo = b == 0 ? -1 : edges[b - 1];
-        if (hi <= lo) { bucket_begin[b] = 0; bucket_count[b] = 0; continue; }
-        const int32_t begin = cursor[hi];
-        const int32_t end = (lo >= 0) ? cursor[lo] : pos;
-        bucket_begin[b] = begin;
-        bucket_count[b] = end - begin;
+    __syncthreads();
+    if (t == 0) {
+        for (int b = 0; b < nb; ++b) {
+            const int hi = edges[b] > max_n ? max_n : edges[b];
+            const int lo = b == 0 ? -1 : edges[b - 1];
+            if (hi <= lo) { bucket_begin[b] = 0; bucket_count[b] = 0; continue; }
+            const int32_t begin = cursor[hi];
+            const int32_t end = (lo >= 0) ? cursor[lo] : total;
+            bucket_begin[b] = begin;
+            bucket_count[b] = end - begin;
+        }
     }
 }
 
@@ -184,7 +211,7 @@ int ragged_typed(int dt, const void* data, const int64_t* offsets, const int32_t
     hipLaunchKernelGGL(rg_zero, dim3(1), dim3(256), 0, s, hist, max_n + 1);
     const int g = (int)((count + 255) / 256 > 1024 ? 1024 : (count + 255) / 256);
     hipLaunchKernelGGL(rg_hist, dim3(g), dim3(256), 0, s, sizes, count, max_n, hist);
-    hipLaunchKernelGGL(rg_scan, dim3(1), dim3(64), 0, s, hist, max_n, edges, ne, cursor, bbegin, bcount);
+    hipLaunchKernelGGL(rg_scan, dim3(1), dim3(256), 0, s, hist, max_n, edges, ne, cursor, bbegin, bcount);
     hipLaunchKernelGGL(rg_scatter, dim3(g), dim3(256), 0, s, sizes, count, max_n, cursor, order);
     int err = (int)hipGetLastError();
     if (err) return err;
