@@ -48,7 +48,7 @@ struct BPState {
 template <typename T>
 struct BPArgs {
     T* W;            // W_old: stale matrix (upper triangle, physical indices)
-    T* P;            // n x (2*nb): U_j at [c*2nb + j], V_j at [c*2nb + nb + j]
+    T* P;            // (2*nb) x n: U_j at [j*n + c], V_j at [(nb + j)*n + c] (pair-major: coalesced in c)
     T* R;            // 2 x n row buffers
     int* perm;       // logical -> physical index into W
     double* part_v;  // per-block argmax partials
@@ -122,47 +122,71 @@ __global__ void __launch_bounds__(BP_THREADS) bp_row(BPArgs<T> a, int k, int i, 
     if (st->dead) return;
     const int64_t n = a.n;
     const int nb = a.nb;
-    const int twonb = 2 * nb;
     const int c = c_lo + blockIdx.x * BP_THREADS + threadIdx.x;
+    const T* PU = a.P;                    // U_j[r] at PU[j*n + r]
+    const T* PV = a.P + (int64_t)nb * n;  // V_j[r] at PV[j*n + r]
     const bool valid = c < n;
     const T* Rk = a.R + (int64_t)((k >> 1) & 1) * n;          // row k (current, post-swap)
     T* Rl = a.R + (int64_t)(((k >> 1) + (DO_M ? 1 : 0)) & 1) * n;  // row kl output
 
+    // One pass over the panel's completed pairs j < i, eight at a time
+    // (nb is a multiple of 8, so the batch never leaves P; lanes j >= i are
+    // masked to zero):
+    //   r1 = sum U_j[k+1] V_j[c]  - V_j[k+1] U_j[c]    (row k+1 at column c)
+    //   rk = sum U_j[k+1] V_j[kl] - V_j[k+1] U_j[kl]   (row k+1 at column kl)
+    //   rl = sum U_j[kl]  V_j[c]  - V_j[kl]  U_j[c]    (row kl at column c)
+    // The [k+1] and [kl] operands are the same for every thread.
+    const int kl = DO_M ? k + 2 : k;
+    const bool live = valid && c > kl;        // column of the new pivot row
+    const int cc = valid ? c : kl;            // a safe column for masked threads
+    T r1 = zero<T>(), rk = zero<T>(), rl = zero<T>();
+    for (int j0 = 0; j0 < i; j0 += 8) {
+        T uc8[8], vc8[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            uc8[t] = PU[(int64_t)(j0 + t) * n + cc];
+            vc8[t] = PV[(int64_t)(j0 + t) * n + cc];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const bool on = j0 + t < i;
+            const T z = zero<T>();
+            const T uc_ = on ? uc8[t] : z, vc_ = on ? vc8[t] : z;
+            const T ul = on ? PU[(int64_t)(j0 + t) * n + kl] : z;
+            const T vl = on ? PV[(int64_t)(j0 + t) * n + kl] : z;
+            rl = rl + ul * vc_ - vl * uc_;
+            if (DO_M) {
+                const T u1 = on ? PU[(int64_t)(j0 + t) * n + k + 1] : z;
+                const T v1 = on ? PV[(int64_t)(j0 + t) * n + k + 1] : z;
+                r1 = r1 + u1 * vc_ - v1 * uc_;
+                rk = rk + u1 * vl - v1 * ul;
+            }
+        }
+    }
+
     T uc = zero<T>(), vc = zero<T>();
     T u_kl = zero<T>(), v_kl = zero<T>();
-    const int kl = DO_M ? k + 2 : k;
     if (DO_M) {
         const T inv = recip(st->piv);
-        const T* Pk1 = a.P + (int64_t)(k + 1) * twonb;
         if (valid && c >= k + 2) {
-            const T* Pc = a.P + (int64_t)c * twonb;
-            T r1 = wp(a.W, a.perm, n, k + 1, c);
-            for (int j = 0; j < i; ++j) r1 = r1 + Pk1[j] * Pc[nb + j] - Pk1[nb + j] * Pc[j];
-            uc = r1 * inv;
+            uc = (wp(a.W, a.perm, n, k + 1, c) + r1) * inv;
             vc = Rk[c];
-            a.P[(int64_t)c * twonb + i] = uc;
-            a.P[(int64_t)c * twonb + nb + i] = vc;
+            a.P[(int64_t)i * n + c] = uc;
+            a.P[(int64_t)(nb + i) * n + c] = vc;
         }
         if (DO_L && kl < n) {
             // U_i[kl], V_i[kl], recomputed by every block
-            const T* Pl = a.P + (int64_t)kl * twonb;
-            T r1 = wp(a.W, a.perm, n, k + 1, kl);
-            for (int j = 0; j < i; ++j) r1 = r1 + Pk1[j] * Pl[nb + j] - Pk1[nb + j] * Pl[j];
-            u_kl = r1 * inv;
+            u_kl = (wp(a.W, a.perm, n, k + 1, kl) + rk) * inv;
             v_kl = Rk[kl];
         }
     }
     if (!DO_L) return;
 
     // row kl at column c > kl
-    const int npairs = DO_M ? i : i;   // completed pairs whose vectors are in P (pair i via registers)
     double best = -1.0;
     int bi = 0x7fffffff;
-    if (valid && c > kl) {
-        const T* Pl = a.P + (int64_t)kl * twonb;
-        const T* Pc = a.P + (int64_t)c * twonb;
-        T r = wp(a.W, a.perm, n, kl, c);
-        for (int j = 0; j < npairs; ++j) r = r + Pl[j] * Pc[nb + j] - Pl[nb + j] * Pc[j];
+    if (live) {
+        T r = wp(a.W, a.perm, n, kl, c) + rl;
         if (DO_M) r = r + u_kl * vc - v_kl * uc;
         Rl[c] = r;
         best = (double)mag2(r);
@@ -191,16 +215,33 @@ __global__ void __launch_bounds__(BP_THREADS) bp_row(BPArgs<T> a, int k, int i, 
     if (!s_last) return;
     __threadfence();
 
-    // last block: final argmax over the partials, then the interchange
+    // last block: final argmax over the partials (all threads load in
+    // parallel), then the interchange
     __shared__ int s_p;
     __shared__ int s_sw;
-    if (threadIdx.x == 0) {
+    {
         double b = -1.0;
         int p = 0x7fffffff;
-        for (unsigned g = 0; g < gridDim.x; ++g) {
+        for (unsigned g = threadIdx.x; g < gridDim.x; g += BP_THREADS) {
             const double v = __hip_atomic_load(&a.part_v[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const int ix = __hip_atomic_load(&a.part_i[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (v > b || (v == b && ix < p)) { b = v; p = ix; }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ob = __shfl_xor(b, off);
+            const int oi = __shfl_xor(p, off);
+            if (ob > b || (ob == b && oi < p)) { b = ob; p = oi; }
+        }
+        __syncthreads();   // s_v / s_i free again
+        if ((threadIdx.x & 63) == 0) { s_v[threadIdx.x >> 6] = b; s_i[threadIdx.x >> 6] = p; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double b = s_v[0];
+        int p = s_i[0];
+        for (int w = 1; w < BP_THREADS / 64; ++w) {
+            if (s_v[w] > b || (s_v[w] == b && s_i[w] < p)) { b = s_v[w]; p = s_i[w]; }
         }
         if (p >= n) p = kl + 1;   // only NaN candidates
         const T piv = Rl[p];
@@ -230,11 +271,11 @@ __global__ void __launch_bounds__(BP_THREADS) bp_row(BPArgs<T> a, int k, int i, 
         // panel vectors of the pairs so far follow the interchange
         const int p = s_p;
         const int cnt = DO_M ? i + 1 : i;
-        T* A1 = a.P + (int64_t)(kl + 1) * twonb;
-        T* A2 = a.P + (int64_t)p * twonb;
         for (int j = threadIdx.x; j < cnt; j += BP_THREADS) {
-            T x = A1[j]; A1[j] = A2[j]; A2[j] = x;
-            T y = A1[nb + j]; A1[nb + j] = A2[nb + j]; A2[nb + j] = y;
+            T* U = a.P + (int64_t)j * n;
+            T* V = a.P + (int64_t)(nb + j) * n;
+            T x = U[kl + 1]; U[kl + 1] = U[p]; U[p] = x;
+            T y = V[kl + 1]; V[kl + 1] = V[p]; V[p] = y;
         }
     }
 }
@@ -258,14 +299,13 @@ __global__ void __launch_bounds__(256) bp_trailing(const T* __restrict__ Wo, T* 
     while (t >= ntile - ti) { t -= ntile - ti; ++ti; }
     const int tj = ti + t;
     const int r0 = lo + ti * BP_TILE, c0 = lo + tj * BP_TILE;
-    const int twonb = 2 * nb;
     for (int e = threadIdx.x; e < BP_TILE * np; e += 256) {
-        const int q = e / np, j = e % np;
+        const int q = e % BP_TILE, j = e / BP_TILE;   // consecutive threads: consecutive rows (coalesced)
         const int r = r0 + q, c = c0 + q;
-        Ur[q * ld + j] = (r < n) ? P[(int64_t)r * twonb + j] : zero<T>();
-        Vr[q * ld + j] = (r < n) ? P[(int64_t)r * twonb + nb + j] : zero<T>();
-        Uc[q * ld + j] = (c < n) ? P[(int64_t)c * twonb + j] : zero<T>();
-        Vc[q * ld + j] = (c < n) ? P[(int64_t)c * twonb + nb + j] : zero<T>();
+        Ur[q * ld + j] = (r < n) ? P[(int64_t)j * n + r] : zero<T>();
+        Vr[q * ld + j] = (r < n) ? P[(int64_t)(nb + j) * n + r] : zero<T>();
+        Uc[q * ld + j] = (c < n) ? P[(int64_t)j * n + c] : zero<T>();
+        Vc[q * ld + j] = (c < n) ? P[(int64_t)(nb + j) * n + c] : zero<T>();
     }
     __syncthreads();
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
@@ -357,6 +397,7 @@ template <typename T>
 int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* log_out, T* phase_out,
                  hipStream_t s) {
     (void)mixed;
+    if (nb <= 0 || nb % 8 != 0) return (int)hipErrorInvalidValue;   // bp_row reads P in batches of 8 pairs
     const BPLayout<T> L = bp_layout<T>(n, nb);
     char* base = (char*)ws;
     T* W[2] = {(T*)(base + L.w0), (T*)(base + L.w1)};
