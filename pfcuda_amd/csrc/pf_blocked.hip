@@ -27,6 +27,9 @@
 // finished by the last block to arrive (a device-scope counter), which also
 // applies the interchange and accumulates log|pivot| and the sign/phase.
 // Only the upper triangle of W is ever read or written.
+#include <cstdlib>
+#include <type_traits>
+
 #include "pf_batch.h"
 #include "pf_blocked.h"
 #include "pf_types.h"
@@ -38,7 +41,7 @@ constexpr int BP_TILE = 64;        // trailing-update tile edge
 
 template <typename T>
 struct BPState {
-    int p, dead, counter, pad;
+    int p, dead, counter, err;
     T piv;
     double lm;
     int le, pad2;
@@ -59,12 +62,19 @@ struct BPArgs {
 };
 
 // Element (r, c) of the stale matrix seen through perm, from the upper triangle.
+// Loads are unconditional (the diagonal when pr == pc) and the result is
+// selected afterwards, so that a run of such reads is issued back to back
+// instead of one branch-and-wait each.
+template <typename T>
+__device__ __forceinline__ T w_at(const T* W, int64_t n, int pr, int pc) {
+    const int a = pr < pc ? pr : pc, b = pr < pc ? pc : pr;
+    const T v = W[(int64_t)a * n + b];
+    return pr < pc ? v : (pr > pc ? -v : zero<T>());
+}
+
 template <typename T>
 __device__ __forceinline__ T wp(const T* W, const int* perm, int64_t n, int r, int c) {
-    const int pr = perm[r], pc = perm[c];
-    if (pr < pc) return W[(int64_t)pr * n + pc];
-    if (pr > pc) return -W[(int64_t)pc * n + pr];
-    return zero<T>();
+    return w_at(W, n, perm[r], perm[c]);
 }
 
 template <typename T>
@@ -91,13 +101,14 @@ __device__ __forceinline__ void accumulate(BPState<T>* st, T piv, bool sw) {
 }
 
 template <typename T>
-__global__ void bp_init(BPArgs<T> a) {
+__global__ void bp_init(BPArgs<T> a, int* bar) {
     const int64_t n = a.n;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         a.perm[i] = (int)i;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         BPState<T>* st = a.st;
-        st->p = 0; st->dead = 0; st->counter = 0;
+        st->p = 0; st->dead = 0; st->counter = 0; st->err = 0;
+        *bar = 0;
         st->piv = one<T>();
         st->lm = 1.0; st->le = 0;
         st->ph_re = 1.0; st->ph_im = 0.0;
@@ -189,8 +200,8 @@ __global__ void __launch_bounds__(BP_THREADS) bp_row(BPArgs<T> a, int k, int i, 
         T r = wp(a.W, a.perm, n, kl, c) + rl;
         if (DO_M) r = r + u_kl * vc - v_kl * uc;
         Rl[c] = r;
-        best = (double)mag2(r);
-        bi = c;
+        const double m = (double)mag2(r);
+        if (m == m) { best = m; bi = c; }   // NaN is never a candidate
     }
     // block argmax
 #pragma unroll
@@ -280,13 +291,315 @@ __global__ void __launch_bounds__(BP_THREADS) bp_row(BPArgs<T> a, int k, int i, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Cooperative panel: one launch factors a whole panel (np pair steps) with
+// one grid barrier per pair step, in place of a launch per pair step.
+//
+// Columns are addressed by "slot", their index in the panel's starting W.
+// Block b owns the slots k0 + 256 b + t (thread t) and keeps their panel
+// vectors U_j, V_j in LDS.  An interchange only changes two maps, kept in
+// LDS by every block and updated identically from the common pivot
+// sequence: logical -> slot for the panel's own positions (Qpan) and
+// slot -> logical for the block's slots (Qi).  Row k of pair i is stored
+// as V_i directly, in slot coordinates.  At the end of the panel the
+// logical -> slot map goes to `perm` for the trailing update, whose P reads
+// are then by slot.
+//
+// After computing the next pivot row, every block publishes its best
+// candidate (|value|, logical index, slot, value) together with that slot's
+// panel entries U_j, V_j; after the barrier every block reduces the
+// partials itself, so the pivot row's correction coefficients arrive in the
+// same round trip as the argmax.  Partials and candidate entries are
+// double-buffered by pair-step parity.
+// ---------------------------------------------------------------------------
+constexpr int BC_T = 256;       // threads per block = slots per block
+constexpr int BC_MAXG = 64;     // blocks at most (one wave reduces the partials)
+constexpr int BC_WAVES = BC_T / 64;
+constexpr int BC_CPW = BC_MAXG / BC_WAVES;   // candidate rows held per wave
+
+template <typename T>
+struct BPPart {
+    double best;
+    int idx, slot;
+    T val;
+};
+
+template <typename T>
+struct BPCoop {
+    BPPart<T>* part;   // [2][BC_MAXG]
+    T* cand;           // [2][BC_MAXG][2*nb]: U_j at j, V_j at nb + j
+    int* bar;          // monotonic barrier counter
+};
+
+__device__ __forceinline__ void bc_barrier(int* bar, int target, int* err) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // this thread's stores are visible device-wide
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int spins = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > (1 << 22)) {   // seconds: the grid is co-resident, so this is a failure
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// Entry (r, c) of the stale matrix in slot coordinates (upper triangle stored).
+template <typename T>
+__device__ __forceinline__ T wsl(const T* W, int64_t n, int r, int c) {
+    return w_at(W, n, r, c);
+}
+
+__device__ __forceinline__ bool bc_better(double b, int i, double bb, int bi) {
+    return b > bb || (b == bb && i < bi);
+}
+
+// Block argmax of (best, logical index) and publication of the winner:
+// partial -> part, its panel entries U_j, V_j (j < cnt) -> cand.
+template <typename T, int NB>
+__device__ __forceinline__ void bc_publish(double best, int bi, T val, int lg, int x, bool own, int cnt,
+                                           BPPart<T>* part, T* cand, const T* Ps, double* r_best, int* r_idx,
+                                           int* r_t) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    double b = best;
+    int ix = bi;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(b, off);
+        const int oi = __shfl_xor(ix, off);
+        if (bc_better(ob, oi, b, ix)) { b = ob; ix = oi; }
+    }
+    if (lane == 0) { r_best[wv] = b; r_idx[wv] = ix; }
+    __syncthreads();
+    double gb = r_best[0];
+    int gi = r_idx[0];
+    for (int u = 1; u < BC_WAVES; ++u)
+        if (bc_better(r_best[u], r_idx[u], gb, gi)) { gb = r_best[u]; gi = r_idx[u]; }
+    const bool found = gb >= 0.0;
+    const bool mine = found && own && best >= 0.0 && lg == gi;
+    if (mine) {
+        *r_t = t;
+        BPPart<T> pt;
+        pt.best = gb; pt.idx = gi; pt.slot = x; pt.val = val;
+        *part = pt;
+    } else if (!found && t == 0) {
+        BPPart<T> pt;
+        pt.best = -1.0; pt.idx = 0x7fffffff; pt.slot = -1; pt.val = zero<T>();
+        *part = pt;
+    }
+    __syncthreads();
+    if (found && t < 2 * NB && (t % NB) < cnt) cand[t] = Ps[t * BC_T + *r_t];
+}
+
+template <typename T, int NB>
+__global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int k0, int np, int bar_base) {
+    extern __shared__ unsigned char smem_raw[];
+    T* Ps = reinterpret_cast<T*>(smem_raw);   // U_j at Ps[j*BC_T + t], V_j at Ps[(NB + j)*BC_T + t]
+    __shared__ int Qpan[2 * NB + 2];           // slot of logical k0 + q
+    __shared__ T B1[2 * NB];                   // winner's entries: U_j, V_j at the new row k+1
+    __shared__ T pre1[2 * NB], pre2[2 * NB];   // entries at the pre-interchange slots of rows k+1, k+2
+    __shared__ double r_best[BC_WAVES];
+    __shared__ int r_idx[BC_WAVES], r_t;
+    __shared__ int s_w, s_p, s_sp;
+    __shared__ T s_piv;
+
+    BPState<T>* st = a.st;
+    if (st->dead) return;
+    const int64_t n = a.n;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int G = gridDim.x;
+    const int x = k0 + blockIdx.x * BC_T + t;   // own slot
+    const bool own = x < n;
+    T* PU = a.P;                                // U_j[x] at PU[j*n + x]
+    T* PV = a.P + (int64_t)NB * n;              // V_j[x] at PV[j*n + x]
+    int nbar = 0;
+
+    double lm = 0.0, ph_re = 0.0, ph_im = 0.0;   // pivot log / phase, kept by block 0 thread 0
+    int le = 0;
+    const bool keeper = blockIdx.x == 0 && t == 0;
+    if (keeper) { lm = st->lm; le = st->le; ph_re = st->ph_re; ph_im = st->ph_im; }
+
+    for (int q = t; q < 2 * NB + 2; q += BC_T) Qpan[q] = k0 + q;
+    int lg = x;                                 // logical index of the own slot
+
+    // first pivot row of the panel: row k0 of W (nothing pending)
+    {
+        double best = -1.0;
+        T r = zero<T>();
+        if (own && x > k0) {
+            r = a.W[(int64_t)k0 * n + x];
+            Ps[NB * BC_T + t] = r;
+            PV[x] = r;
+            const double m = (double)mag2(r);
+            if (m == m) best = m;
+        }
+        bc_publish<T, NB>(best, lg, r, lg, x, own, 0, &co.part[blockIdx.x], co.cand + (int64_t)blockIdx.x * 2 * NB,
+                          Ps, r_best, r_idx, &r_t);
+        ++nbar;
+        bc_barrier(co.bar, bar_base + nbar * G, &st->err);
+    }
+
+    for (int i = 0; i < np; ++i) {
+        const int k = k0 + 2 * i;
+        const bool has_next = i + 1 < np;       // row k+2 is this panel's next pivot row
+        const int par = i & 1;
+        // ---- round 1: partials, every block's candidate entries, entries at the
+        //      pre-interchange slots of rows k+1 and k+2, and row k+2 of W
+        const int sa = Qpan[k + 1 - k0];
+        const int sb = has_next ? Qpan[k + 2 - k0] : -1;
+        // (every load below is unconditional from a valid address, masked after)
+        BPPart<T> pp = co.part[par * BC_MAXG + (t < G ? t : 0)];
+        if (t >= G) { pp.best = -1.0; pp.idx = 0x7fffffff; }
+        T cv[BC_CPW];
+        {
+            const bool lane_ok = lane < 2 * NB && (lane % NB) < i;
+            const T* cb = co.cand + (int64_t)par * BC_MAXG * 2 * NB;
+#pragma unroll
+            for (int mm = 0; mm < BC_CPW; ++mm) {
+                const int g = wv + BC_WAVES * mm;
+                const bool ok = g < G && lane_ok;
+                const T v = cb[ok ? (int64_t)g * 2 * NB + lane : 0];
+                cv[mm] = ok ? v : zero<T>();
+            }
+        }
+        const int xs = own ? x : k0;           // a valid slot for masked reads
+        const int sbs = sb >= 0 ? sb : sa;
+        {
+            const int tt = t & (2 * NB - 1);
+            const int j = tt % NB;
+            const bool isv = tt >= NB;
+            const bool ok = j < i || (isv && j == i);
+            const T* src = (isv ? PV : PU) + (int64_t)j * n;
+            const T v1 = src[sa], v2 = src[sbs];
+            if (t < 2 * NB) {
+                pre1[t] = ok ? v1 : zero<T>();
+                pre2[t] = (ok && sb >= 0) ? v2 : zero<T>();
+            }
+        }
+        T w2 = wsl(a.W, n, sbs, xs);
+        if (!(has_next && own && lg >= k + 1)) w2 = zero<T>();
+        if (wv == 0) {
+            double b = pp.best;
+            int bi = pp.idx, bw = t;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_xor(b, off);
+                const int oi = __shfl_xor(bi, off);
+                const int ow = __shfl_xor(bw, off);
+                if (bc_better(ob, oi, b, bi)) { b = ob; bi = oi; bw = ow; }
+            }
+            if (t == 0) s_w = (b >= 0.0) ? bw : -1;
+        }
+        __syncthreads();
+        const int wb = s_w;
+        if (wb >= 0 && t == wb) { s_p = pp.idx; s_sp = pp.slot; s_piv = pp.val; }
+        if (wb >= 0 && wv == (wb % BC_WAVES) && lane < 2 * NB) {
+            const int mw = wb / BC_WAVES;
+            T v = zero<T>();
+#pragma unroll
+            for (int mm = 0; mm < BC_CPW; ++mm)
+                if (mm == mw) v = cv[mm];
+            B1[lane] = v;
+        }
+        __syncthreads();
+        int p, sp;
+        T piv;
+        if (wb >= 0) { p = s_p; sp = s_sp; piv = s_piv; }
+        else { p = k + 1; sp = sa; piv = pre1[NB + i]; }   // no finite candidate (NaN row)
+        if (is_zero(piv)) {                                // same decision in every block
+            if (keeper) st->dead = 1;
+            return;
+        }
+        const bool sw = p != k + 1;
+        if (keeper) {
+            BPState<T> tmp;
+            tmp.lm = lm; tmp.le = le; tmp.ph_re = ph_re; tmp.ph_im = ph_im;
+            accumulate(&tmp, piv, sw);
+            lm = tmp.lm; le = tmp.le; ph_re = tmp.ph_re; ph_im = tmp.ph_im;
+        }
+        // ---- interchange k+1 <-> p
+        __syncthreads();                                   // every thread has read Qpan
+        if (sw && t == 0) {
+            Qpan[k + 1 - k0] = sp;
+            if (p - k0 < 2 * NB + 2) Qpan[p - k0] = sa;
+        }
+        if (x == sp) lg = k + 1;
+        else if (sw && x == sa) lg = p;
+        const int y1 = sp;
+        const int y2 = (p == k + 2) ? sa : sb;
+        const T* b1 = (wb >= 0) ? B1 : pre1;
+        const T* b2 = (p == k + 2) ? pre1 : pre2;
+        if (k + 2 >= n) break;                             // last pair of the matrix
+
+        // ---- round 2: row k+1 of W at the own slots
+        const bool act1 = own && lg >= k + 2;
+        const int y2s = has_next ? y2 : y1;
+        T w1 = wsl(a.W, n, y1, xs);
+        if (!act1) w1 = zero<T>();
+        T w12 = wsl(a.W, n, y1, y2s);
+        if (!has_next) w12 = zero<T>();
+        if (has_next && p == k + 2) {
+            w2 = wsl(a.W, n, y2, xs);
+            if (!(own && lg >= k + 3)) w2 = zero<T>();
+        }
+
+        const T inv = recip(piv);
+        T r1 = zero<T>(), rl = zero<T>(), rk = zero<T>();
+#pragma unroll 4
+        for (int j = 0; j < i; ++j) {
+            const T uj = Ps[j * BC_T + t], vj = Ps[(NB + j) * BC_T + t];
+            const T b1u = b1[j], b1v = b1[NB + j];
+            r1 = r1 + b1u * vj - b1v * uj;
+            if (has_next) {
+                const T b2u = b2[j], b2v = b2[NB + j];
+                rl = rl + b2u * vj - b2v * uj;
+                rk = rk + b1u * b2v - b1v * b2u;
+            }
+        }
+        T u = zero<T>();
+        if (act1) {
+            u = (w1 + r1) * inv;
+            Ps[i * BC_T + t] = u;
+            PU[(int64_t)i * n + x] = u;
+        }
+        if (!has_next) break;
+
+        // ---- row k+2 (the next pivot row) at the own slots -> V_{i+1}
+        const T u_kl = (w12 + rk) * inv;   // U_i at row k+2
+        const T v_kl = b2[NB + i];         // V_i at row k+2
+        double best = -1.0;
+        T r = zero<T>();
+        if (own && lg >= k + 3) {
+            r = w2 + rl + u_kl * Ps[(NB + i) * BC_T + t] - v_kl * u;
+            Ps[(NB + i + 1) * BC_T + t] = r;
+            PV[(int64_t)(i + 1) * n + x] = r;
+            const double m = (double)mag2(r);
+            if (m == m) best = m;
+        }
+        const int npar = (i + 1) & 1;
+        bc_publish<T, NB>(best, lg, r, lg, x, own, i + 1, &co.part[npar * BC_MAXG + blockIdx.x],
+                          co.cand + ((int64_t)npar * BC_MAXG + blockIdx.x) * 2 * NB, Ps, r_best, r_idx, &r_t);
+        ++nbar;
+        bc_barrier(co.bar, bar_base + nbar * G, &st->err);
+    }
+
+    if (own) a.perm[lg] = x;   // logical -> slot, for the trailing update
+    if (keeper) { st->lm = lm; st->le = le; st->ph_re = ph_re; st->ph_im = ph_im; }
+}
+
 // Trailing update over upper tiles of the block [lo, n)^2:
-// Wn[r][c] = Wo[perm r][perm c] + sum_{j<np} (U_j[r] V_j[c] - V_j[r] U_j[c]),  r < c.
+// Wn[r][c] = Wo[perm r][perm c] + sum_{j<np} (U_j[s r] V_j[s c] - V_j[s r] U_j[s c]),  r < c,
+// with s = perm when the panel vectors are stored by slot (pslot), else identity.
 // 256 threads; thread (tx, ty) owns rows r0+ty+16a, cols c0+tx+16b (a, b < 4).
 template <typename T>
 __global__ void __launch_bounds__(256) bp_trailing(const T* __restrict__ Wo, T* __restrict__ Wn,
                                                     const T* __restrict__ P, const int* __restrict__ perm,
-                                                    int64_t n, int lo, int nb, int np, int ntile) {
+                                                    int64_t n, int lo, int nb, int np, int ntile, int pslot) {
     extern __shared__ unsigned char smem_raw[];
     T* sm = reinterpret_cast<T*>(smem_raw);
     const int ld = np + 1;                       // padded row length
@@ -302,10 +615,12 @@ __global__ void __launch_bounds__(256) bp_trailing(const T* __restrict__ Wo, T* 
     for (int e = threadIdx.x; e < BP_TILE * np; e += 256) {
         const int q = e % BP_TILE, j = e / BP_TILE;   // consecutive threads: consecutive rows (coalesced)
         const int r = r0 + q, c = c0 + q;
-        Ur[q * ld + j] = (r < n) ? P[(int64_t)j * n + r] : zero<T>();
-        Vr[q * ld + j] = (r < n) ? P[(int64_t)(nb + j) * n + r] : zero<T>();
-        Uc[q * ld + j] = (c < n) ? P[(int64_t)j * n + c] : zero<T>();
-        Vc[q * ld + j] = (c < n) ? P[(int64_t)(nb + j) * n + c] : zero<T>();
+        const int pr = (r < n) ? (pslot ? perm[r] : r) : 0;
+        const int pc = (c < n) ? (pslot ? perm[c] : c) : 0;
+        Ur[q * ld + j] = (r < n) ? P[(int64_t)j * n + pr] : zero<T>();
+        Vr[q * ld + j] = (r < n) ? P[(int64_t)(nb + j) * n + pr] : zero<T>();
+        Uc[q * ld + j] = (c < n) ? P[(int64_t)j * n + pc] : zero<T>();
+        Vc[q * ld + j] = (c < n) ? P[(int64_t)(nb + j) * n + pc] : zero<T>();
     }
     __syncthreads();
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
@@ -325,18 +640,100 @@ __global__ void __launch_bounds__(256) bp_trailing(const T* __restrict__ Wo, T* 
 #pragma unroll
             for (int y = 0; y < 4; ++y) acc[x][y] = acc[x][y] + ur[x] * vc[y] - vr[x] * uc[y];
     }
+    // stale entries: all sixteen gathers issued before any store
+    const int n1 = (int)n - 1;
+    int pr[4], pc[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) pr[x] = perm[min(r0 + ty + 16 * x, n1)];
+#pragma unroll
+    for (int y = 0; y < 4; ++y) pc[y] = perm[min(c0 + tx + 16 * y, n1)];
+    T wv[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) wv[x][y] = w_at(Wo, n, pr[x], pc[y]);
 #pragma unroll
     for (int x = 0; x < 4; ++x) {
         const int r = r0 + ty + 16 * x;
-        if (r >= n) continue;
-        const int pr = perm[r];
 #pragma unroll
         for (int y = 0; y < 4; ++y) {
             const int c = c0 + tx + 16 * y;
-            if (c >= n || c <= r) continue;
-            const int pc = perm[c];
-            T w = (pr < pc) ? Wo[(int64_t)pr * n + pc] : -Wo[(int64_t)pc * n + pr];
-            Wn[(int64_t)r * n + c] = w + acc[x][y];
+            if (r < n && c < n && c > r) Wn[(int64_t)r * n + c] = wv[x][y] + acc[x][y];
+        }
+    }
+}
+
+// FP64 trailing update on the matrix cores: the same sum as bp_trailing, as
+// the product  C += A B  with  A[r][q] = P[q][s r]  (rows U_0..U_{nb-1},
+// V_0..V_{nb-1} of the panel store) and  B[q][c] = [V_0..V_{nb-1},
+// -U_0..-U_{nb-1}][s c]  (K = 2 nb), per 64 x 64 upper tile.  Wave w owns
+// tile rows 16 w .. 16 w + 15 and four 16 x 16 accumulators across the
+// columns, fed by v_mfma_f64_16x16x4f64 (A operand: lane l holds
+// A[l % 16][l / 16], B operand B[l / 16][l % 16], accumulator register e holds
+// (l / 16 + 4 e, l % 16)).  Operands come straight from P (small, L2
+// resident) with no LDS, so many waves per CU hide the latency of the
+// stale-matrix gather; the accumulators start from the gathered entries,
+// so W_old is read once and W_new written once.
+typedef double bt_d4 __attribute__((ext_vector_type(4)));
+
+template <int NB>
+__global__ void __launch_bounds__(256) bp_trailing_mfma(const double* __restrict__ Wo, double* __restrict__ Wn,
+                                                        const double* __restrict__ P, const int* __restrict__ perm,
+                                                        int64_t n, int lo, int np, int ntile, int pslot) {
+    constexpr int K = 2 * NB;
+    int tb = blockIdx.x, ti = 0;
+    while (tb >= ntile - ti) { tb -= ntile - ti; ++ti; }
+    const int tj = ti + tb;
+    const int r0 = lo + ti * BP_TILE + 16 * (threadIdx.x >> 6), c0 = lo + tj * BP_TILE;
+    const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+    const int n1 = (int)n - 1;
+    // order of the loads: perm, then the L2-resident operands, then the stale
+    // entries from HBM.  Loads complete in issue order, so the products can
+    // start as soon as the operands are in while the gathers are in flight.
+    int pr[4], pc[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) pr[e] = perm[min(r0 + lq + 4 * e, n1)];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pc[u] = perm[min(c0 + 16 * u + lr, n1)];
+    const int pa = perm[min(r0 + lr, n1)];
+    const int sa = pslot ? pa : min(r0 + lr, n1);   // slot of this lane's A row
+    int sb[4];                                       // slots of its B columns
+#pragma unroll
+    for (int u = 0; u < 4; ++u) sb[u] = pslot ? pc[u] : min(c0 + 16 * u + lr, n1);
+    double av[K / 4], bv[K / 4][4];
+#pragma unroll
+    for (int g = 0; g < K / 4; ++g) {
+        const int q = 4 * g + lq;                  // this lane's K index
+        const bool on = (q % NB) < np;
+        const int qb = q < NB ? NB + q : q - NB;   // B row: V_q, then U_{q - nb}
+        const double sg = q < NB ? 1.0 : -1.0;
+        const double va = P[(int64_t)q * n + sa];
+        av[g] = on ? va : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double vb = P[(int64_t)qb * n + sb[u]];
+            bv[g][u] = on ? sg * vb : 0.0;
+        }
+    }
+    double wv[4][4];   // stale entries (r0 + lq + 4e, c0 + 16u + lr)
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) wv[u][e] = w_at(Wo, n, pr[e], pc[u]);   // clamped indices are valid
+    bt_d4 acc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] = bt_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int g = 0; g < K / 4; ++g) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g], bv[g][u], acc[u], 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = r0 + lq + 4 * e, c = c0 + 16 * u + lr;
+            if (c < n && c > r) Wn[(int64_t)r * n + c] = wv[u][e] + acc[u][e];
         }
     }
 }
@@ -344,6 +741,7 @@ __global__ void __launch_bounds__(256) bp_trailing(const T* __restrict__ Wo, T* 
 template <typename T>
 __global__ void bp_finish(const BPState<T>* st, double* log_out, T* phase_out) {
     if (threadIdx.x != 0) return;
+    if (st->err) { *log_out = __builtin_nan(""); *phase_out = nan_of<T>(); return; }
     if (st->dead) { *log_out = -INFINITY; *phase_out = zero<T>(); return; }
     *log_out = log(st->lm) + (double)st->le * 0.69314718055994530942;
     T ph;
@@ -356,7 +754,7 @@ static size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 
 template <typename T>
 struct BPLayout {
-    size_t w0, w1, p, r, perm, pv, pi, st, total;
+    size_t w0, w1, p, r, perm, pv, pi, st, part, cand, bar, total;
 };
 
 template <typename T>
@@ -372,13 +770,50 @@ static BPLayout<T> bp_layout(int64_t n, int nb) {
     L.pv = o; o = al(o + sizeof(double) * nblk);
     L.pi = o; o = al(o + sizeof(int) * nblk);
     L.st = o; o = al(o + sizeof(BPState<T>));
+    L.part = o; o = al(o + sizeof(BPPart<T>) * 2 * BC_MAXG);
+    L.cand = o; o = al(o + sizeof(T) * 2 * BC_MAXG * 2 * nb);
+    L.bar = o; o = al(o + sizeof(int));
     L.total = o;
     return L;
 }
 
+// 32 pair steps per panel; 16 for complex128, whose panel vectors would
+// otherwise not fit the cooperative panel kernel's LDS.
 int blocked_panel_width(int dtype) {
-    (void)dtype;
-    return 32;
+    return dtype == DT_C128 ? 16 : 32;
+}
+
+// The cooperative panel kernel runs when the device supports cooperative
+// launches (PFH_BLOCKED_COOP=0 turns it off) and the panel's columns fit
+// BC_MAXG blocks of BC_T slots.
+static bool coop_enabled() {
+    static int v = -1;
+    if (v < 0) {
+        const char* e = std::getenv("PFH_BLOCKED_COOP");
+        int dev = 0, ok = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ok, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess)
+            ok = 0;
+        v = (ok && !(e && e[0] == '0')) ? 1 : 0;
+    }
+    return v == 1;
+}
+
+template <typename T, int NB>
+static hipError_t launch_panel(const BPArgs<T>& a, const BPCoop<T>& co, int k0, int np, int bar_base, int G,
+                               hipStream_t s) {
+    const size_t smem = sizeof(T) * 2 * NB * BC_T;
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void*)bp_panel<T, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)smem);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    BPArgs<T> a_ = a;
+    BPCoop<T> co_ = co;
+    void* args[] = {&a_, &co_, &k0, &np, &bar_base};
+    return hipLaunchCooperativeKernel((const void*)bp_panel<T, NB>, dim3(G), dim3(BC_T), args, (unsigned)smem, s);
 }
 
 size_t blocked_workspace_bytes(int dtype, int64_t n, bool mixed) {
@@ -410,15 +845,29 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
     a.st = (BPState<T>*)(base + L.st);
     a.n = n;
     a.nb = nb;
+    BPCoop<T> co{};
+    co.part = (BPPart<T>*)(base + L.part);
+    co.cand = (T*)(base + L.cand);
+    co.bar = (int*)(base + L.bar);
+    const bool coop = coop_enabled() && (nb == 16 || nb == 32);
     hipError_t e = hipMemcpyAsync(W[0], A, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return (int)e;
     a.W = W[0];
-    hipLaunchKernelGGL(bp_init<T>, dim3(64), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(bp_init<T>, dim3(64), dim3(256), 0, s, a, co.bar);
     int cur = 0;
+    int bar_base = 0;
     auto blocks = [&](int64_t c_lo) { return (unsigned)((n - c_lo + BP_THREADS - 1) / BP_THREADS); };
     for (int64_t k0 = 0; k0 + 1 < n; k0 += 2 * nb) {
         const int np = (int)((n - k0) / 2 < nb ? (n - k0) / 2 : nb);
         a.W = W[cur];
+        const int G = (int)((n - k0 + BC_T - 1) / BC_T);
+        const bool panel_coop = coop && G <= BC_MAXG;
+        if (panel_coop) {
+            e = (nb == 32) ? launch_panel<T, 32>(a, co, (int)k0, np, bar_base, G, s)
+                           : launch_panel<T, 16>(a, co, (int)k0, np, bar_base, G, s);
+            if (e != hipSuccess) return (int)e;
+            bar_base += np * G;
+        } else {
         // first pivot row of the panel
         hipLaunchKernelGGL((bp_row<T, false, true>), dim3(blocks(k0 + 1)), dim3(BP_THREADS), 0, s, a, (int)k0, 0,
                            (int)(k0 + 1));
@@ -433,6 +882,7 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
                                    (int)k, i, (int)(k + 2));
             }
         }
+        }
         const int64_t lo = k0 + 2 * np;
         if (lo + 1 < n) {
             const int ntile = (int)((n - lo + BP_TILE - 1) / BP_TILE);
@@ -443,8 +893,18 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
                                         (int)smem);
                 if (e != hipSuccess) return (int)e;
             }
-            hipLaunchKernelGGL(bp_trailing<T>, dim3(tiles), dim3(256), smem, s, W[cur], W[cur ^ 1], a.P, a.perm,
-                               n, (int)lo, nb, np, ntile);
+            if constexpr (std::is_same<T, double>::value) {
+                if (nb == 32) {
+                    hipLaunchKernelGGL(bp_trailing_mfma<32>, dim3(tiles), dim3(256), 0, s, W[cur], W[cur ^ 1], a.P,
+                                       a.perm, n, (int)lo, np, ntile, panel_coop ? 1 : 0);
+                } else {
+                    hipLaunchKernelGGL(bp_trailing<T>, dim3(tiles), dim3(256), smem, s, W[cur], W[cur ^ 1], a.P,
+                                       a.perm, n, (int)lo, nb, np, ntile, panel_coop ? 1 : 0);
+                }
+            } else {
+                hipLaunchKernelGGL(bp_trailing<T>, dim3(tiles), dim3(256), smem, s, W[cur], W[cur ^ 1], a.P, a.perm,
+                                   n, (int)lo, nb, np, ntile, panel_coop ? 1 : 0);
+            }
             hipLaunchKernelGGL(bp_reset_perm<T>, dim3(64), dim3(256), 0, s, a.perm, lo, n);
             cur ^= 1;
         }

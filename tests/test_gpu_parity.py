@@ -184,6 +184,24 @@ def test_blocked_zero_and_identity(pf, cuda):
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
+def test_blocked_dies_mid_panel(pf, cuda, dtype):
+    # 100 zero rows/columns scattered by a permutation: the first 250 pair
+    # steps have nonzero pivots, then the trailing block is exactly zero, so
+    # the factorization stops inside a panel with (-inf, 0).
+    rng = np.random.default_rng(77)
+    n = 600
+    A = oracle.skew_gaussian(rng, n, complex_=dtype.is_complex)
+    A[500:, :] = 0
+    A[:, 500:] = 0
+    perm = rng.permutation(n)
+    A = np.ascontiguousarray(A[perm][:, perm])
+    la, s = pf.slog_pfaffian(torch.from_numpy(A).to(cuda), method=pf.PFH_METHOD_BLOCKED)
+    assert float(la) == -math.inf and complex(s.item()) == 0
+    l0, p0 = oracle.slog(A)
+    assert l0 == -math.inf and p0 == 0
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
 @pytest.mark.parametrize("n", [8, 32, 64, 128, 200, 300])
 def test_grad_is_half_inverse_transpose(pf, cuda, dtype, n):
     from pfcuda_amd.adjoint import slog_grad
