@@ -33,11 +33,16 @@
 #include "pf_batch.h"
 #include "pf_blocked.h"
 #include "pf_types.h"
+#include "pf_wave.h"
 
 namespace pfh {
 
 constexpr int BP_THREADS = 128;    // columns per block in the row kernels
 constexpr int BP_TILE = 64;        // trailing-update tile edge
+constexpr int BC_T = 256;          // cooperative panel: threads per block = slots per block
+constexpr int BC_MAXG = 64;        // cooperative panel: blocks at most (one wave reduces the partials)
+constexpr int BC_WAVES = BC_T / 64;
+constexpr int BC_CPW = BC_MAXG / BC_WAVES;   // candidate rows held per wave
 
 template <typename T>
 struct BPState {
@@ -77,38 +82,45 @@ __device__ __forceinline__ T wp(const T* W, const int* perm, int64_t n, int r, i
     return w_at(W, n, perm[r], perm[c]);
 }
 
+// log|pivot| (as mantissa lm and exponent le) and the sign / phase, with a
+// sign flip for an interchange
 template <typename T>
-__device__ __forceinline__ void accumulate(BPState<T>* st, T piv, bool sw) {
-    double m = st->lm * absd(piv);
+__device__ __forceinline__ void accumulate(double& lm, int& le, double& ph_re, double& ph_im, T piv, bool sw) {
+    double m = lm * absd(piv);
     int ex;
     m = frexp(m, &ex);
-    st->lm = m;
-    st->le += ex;
+    lm = m;
+    le += ex;
     if constexpr (is_cplx<T>::value) {
         const double a = hypot((double)piv.re, (double)piv.im);
         const double pr = (double)piv.re / a, pi = (double)piv.im / a;
-        double nr = st->ph_re * pr - st->ph_im * pi, ni = st->ph_re * pi + st->ph_im * pr;
+        double nr = ph_re * pr - ph_im * pi, ni = ph_re * pi + ph_im * pr;
         const double nm = hypot(nr, ni);
         nr /= nm; ni /= nm;
         if (sw) { nr = -nr; ni = -ni; }
-        st->ph_re = nr; st->ph_im = ni;
+        ph_re = nr; ph_im = ni;
     } else {
-        double s = st->ph_re;
+        double s = ph_re;
         if (piv < T(0)) s = -s;
         if (sw) s = -s;
-        st->ph_re = s;
+        ph_re = s;
     }
 }
 
 template <typename T>
-__global__ void bp_init(BPArgs<T> a, int* bar) {
+__device__ __forceinline__ void accumulate(BPState<T>* st, T piv, bool sw) {
+    accumulate(st->lm, st->le, st->ph_re, st->ph_im, piv, sw);
+}
+
+template <typename T>
+__global__ void bp_init(BPArgs<T> a, int* flags) {
     const int64_t n = a.n;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         a.perm[i] = (int)i;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         BPState<T>* st = a.st;
         st->p = 0; st->dead = 0; st->counter = 0; st->err = 0;
-        *bar = 0;
+        for (int g = 0; g < BC_MAXG; ++g) flags[g] = 0;
         st->piv = one<T>();
         st->lm = 1.0; st->le = 0;
         st->ph_re = 1.0; st->ph_im = 0.0;
@@ -312,10 +324,6 @@ __global__ void __launch_bounds__(BP_THREADS) bp_row(BPArgs<T> a, int k, int i, 
 // same round trip as the argmax.  Partials and candidate entries are
 // double-buffered by pair-step parity.
 // ---------------------------------------------------------------------------
-constexpr int BC_T = 256;       // threads per block = slots per block
-constexpr int BC_MAXG = 64;     // blocks at most (one wave reduces the partials)
-constexpr int BC_WAVES = BC_T / 64;
-constexpr int BC_CPW = BC_MAXG / BC_WAVES;   // candidate rows held per wave
 
 template <typename T>
 struct BPPart {
@@ -327,22 +335,27 @@ struct BPPart {
 template <typename T>
 struct BPCoop {
     BPPart<T>* part;   // [2][BC_MAXG]
-    T* cand;           // [2][BC_MAXG][2*nb]: U_j at j, V_j at nb + j
-    int* bar;          // monotonic barrier counter
+    T* cand;           // [2][BC_MAXG][2*nb]: U_j at 2j, V_j at 2j + 1
+    int* flags;        // [BC_MAXG]: the last barrier epoch each block reached
 };
 
-__device__ __forceinline__ void bc_barrier(int* bar, int target, int* err) {
+// Grid barrier: every block stores the epoch into its own flag (no shared
+// counter), and one wave polls all flags with one load per lane.
+__device__ __forceinline__ void bc_barrier(int* flags, int epoch, int* err) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // this thread's stores are visible device-wide
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int spins = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            __builtin_amdgcn_s_sleep(1);
-            if (++spins > (1 << 22)) {   // seconds: the grid is co-resident, so this is a failure
-                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) __hip_atomic_store(&flags[blockIdx.x], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x < 64) {
+        const int g = threadIdx.x;
+        for (int spins = 0;; ++spins) {
+            int f = epoch;
+            if (g < (int)gridDim.x) f = __hip_atomic_load(&flags[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__all(f >= epoch)) break;
+            if (spins > (1 << 22)) {   // seconds: the grid is co-resident, so this is a failure
+                if (g == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
+            __builtin_amdgcn_s_sleep(1);
         }
     }
     __syncthreads();
@@ -366,14 +379,8 @@ __device__ __forceinline__ void bc_publish(double best, int bi, T val, int lg, i
                                            BPPart<T>* part, T* cand, const T* Ps, double* r_best, int* r_idx,
                                            int* r_t) {
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    double b = best;
-    int ix = bi;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const double ob = __shfl_xor(b, off);
-        const int oi = __shfl_xor(ix, off);
-        if (bc_better(ob, oi, b, ix)) { b = ob; ix = oi; }
-    }
+    const double b = wave_max(best);                                    // inputs >= -1, no NaN
+    const int ix = wave_min((best == b && best >= 0.0) ? bi : 0x7fffffff);   // ties: lowest index
     if (lane == 0) { r_best[wv] = b; r_idx[wv] = ix; }
     __syncthreads();
     double gb = r_best[0];
@@ -393,16 +400,22 @@ __device__ __forceinline__ void bc_publish(double best, int bi, T val, int lg, i
         *part = pt;
     }
     __syncthreads();
-    if (found && t < 2 * NB && (t % NB) < cnt) cand[t] = Ps[t * BC_T + *r_t];
+    if (found && t < 2 * NB && (t >> 1) < cnt) cand[t] = Ps[2 * ((t >> 1) * BC_T + *r_t) + (t & 1)];
 }
 
 template <typename T, int NB>
-__global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int k0, int np, int bar_base) {
+__global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int k0, int np, int epoch0) {
     extern __shared__ unsigned char smem_raw[];
-    T* Ps = reinterpret_cast<T*>(smem_raw);   // U_j at Ps[j*BC_T + t], V_j at Ps[(NB + j)*BC_T + t]
+    // Panel entries of the own slot, interleaved so that one 16-byte LDS read
+    // returns a pair: U_j at Ps[2*(j*BC_T + t)], V_j at Ps[2*(j*BC_T + t) + 1].
+    T* Ps = reinterpret_cast<T*>(smem_raw);
     __shared__ int Qpan[2 * NB + 2];           // slot of logical k0 + q
-    __shared__ T B1[2 * NB];                   // winner's entries: U_j, V_j at the new row k+1
-    __shared__ T pre1[2 * NB], pre2[2 * NB];   // entries at the pre-interchange slots of rows k+1, k+2
+    // entries at one slot, interleaved (U_j at 2j, V_j at 2j + 1): the
+    // winner's (the new row k+1), and those at the pre-interchange slots of
+    // rows k+1 and k+2
+    // (one array, so that choosing one of them is an index, not a select
+    // between two loads): Bx[0] winner, Bx[1] pre-interchange k+1, Bx[2] k+2
+    __shared__ __align__(16) T Bx[3][2 * NB];
     __shared__ double r_best[BC_WAVES];
     __shared__ int r_idx[BC_WAVES], r_t;
     __shared__ int s_w, s_p, s_sp;
@@ -426,6 +439,7 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
 
     for (int q = t; q < 2 * NB + 2; q += BC_T) Qpan[q] = k0 + q;
     int lg = x;                                 // logical index of the own slot
+    for (int j = 0; j < NB; ++j) { Ps[2 * (j * BC_T + t)] = zero<T>(); Ps[2 * (j * BC_T + t) + 1] = zero<T>(); }
 
     // first pivot row of the panel: row k0 of W (nothing pending)
     {
@@ -433,7 +447,7 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         T r = zero<T>();
         if (own && x > k0) {
             r = a.W[(int64_t)k0 * n + x];
-            Ps[NB * BC_T + t] = r;
+            Ps[2 * t + 1] = r;
             PV[x] = r;
             const double m = (double)mag2(r);
             if (m == m) best = m;
@@ -441,7 +455,7 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         bc_publish<T, NB>(best, lg, r, lg, x, own, 0, &co.part[blockIdx.x], co.cand + (int64_t)blockIdx.x * 2 * NB,
                           Ps, r_best, r_idx, &r_t);
         ++nbar;
-        bc_barrier(co.bar, bar_base + nbar * G, &st->err);
+        bc_barrier(co.flags, epoch0 + nbar, &st->err);
     }
 
     for (int i = 0; i < np; ++i) {
@@ -457,7 +471,7 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         if (t >= G) { pp.best = -1.0; pp.idx = 0x7fffffff; }
         T cv[BC_CPW];
         {
-            const bool lane_ok = lane < 2 * NB && (lane % NB) < i;
+            const bool lane_ok = lane < 2 * NB && (lane >> 1) < i;
             const T* cb = co.cand + (int64_t)par * BC_MAXG * 2 * NB;
 #pragma unroll
             for (int mm = 0; mm < BC_CPW; ++mm) {
@@ -471,29 +485,23 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         const int sbs = sb >= 0 ? sb : sa;
         {
             const int tt = t & (2 * NB - 1);
-            const int j = tt % NB;
-            const bool isv = tt >= NB;
+            const int j = tt >> 1;
+            const bool isv = tt & 1;
             const bool ok = j < i || (isv && j == i);
             const T* src = (isv ? PV : PU) + (int64_t)j * n;
             const T v1 = src[sa], v2 = src[sbs];
             if (t < 2 * NB) {
-                pre1[t] = ok ? v1 : zero<T>();
-                pre2[t] = (ok && sb >= 0) ? v2 : zero<T>();
+                Bx[1][t] = ok ? v1 : zero<T>();
+                Bx[2][t] = (ok && sb >= 0) ? v2 : zero<T>();
             }
         }
         T w2 = wsl(a.W, n, sbs, xs);
         if (!(has_next && own && lg >= k + 1)) w2 = zero<T>();
         if (wv == 0) {
-            double b = pp.best;
-            int bi = pp.idx, bw = t;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ob = __shfl_xor(b, off);
-                const int oi = __shfl_xor(bi, off);
-                const int ow = __shfl_xor(bw, off);
-                if (bc_better(ob, oi, b, bi)) { b = ob; bi = oi; bw = ow; }
-            }
-            if (t == 0) s_w = (b >= 0.0) ? bw : -1;
+            const double b = wave_max(pp.best);
+            const int bi = wave_min((pp.best == b && b >= 0.0) ? pp.idx : 0x7fffffff);
+            const unsigned long long hit = __ballot(b >= 0.0 && pp.best == b && pp.idx == bi);
+            if (t == 0) s_w = hit ? (int)__builtin_ctzll(hit) : -1;
         }
         __syncthreads();
         const int wb = s_w;
@@ -504,24 +512,19 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
 #pragma unroll
             for (int mm = 0; mm < BC_CPW; ++mm)
                 if (mm == mw) v = cv[mm];
-            B1[lane] = v;
+            Bx[0][lane] = v;
         }
         __syncthreads();
         int p, sp;
         T piv;
         if (wb >= 0) { p = s_p; sp = s_sp; piv = s_piv; }
-        else { p = k + 1; sp = sa; piv = pre1[NB + i]; }   // no finite candidate (NaN row)
+        else { p = k + 1; sp = sa; piv = Bx[1][2 * i + 1]; }   // no finite candidate (NaN row)
         if (is_zero(piv)) {                                // same decision in every block
             if (keeper) st->dead = 1;
             return;
         }
         const bool sw = p != k + 1;
-        if (keeper) {
-            BPState<T> tmp;
-            tmp.lm = lm; tmp.le = le; tmp.ph_re = ph_re; tmp.ph_im = ph_im;
-            accumulate(&tmp, piv, sw);
-            lm = tmp.lm; le = tmp.le; ph_re = tmp.ph_re; ph_im = tmp.ph_im;
-        }
+        if (keeper) accumulate(lm, le, ph_re, ph_im, piv, sw);
         // ---- interchange k+1 <-> p
         __syncthreads();                                   // every thread has read Qpan
         if (sw && t == 0) {
@@ -532,8 +535,8 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         else if (sw && x == sa) lg = p;
         const int y1 = sp;
         const int y2 = (p == k + 2) ? sa : sb;
-        const T* b1 = (wb >= 0) ? B1 : pre1;
-        const T* b2 = (p == k + 2) ? pre1 : pre2;
+        const T* b1 = Bx[(wb >= 0) ? 0 : 1];
+        const T* b2 = Bx[(p == k + 2) ? 1 : 2];
         if (k + 2 >= n) break;                             // last pair of the matrix
 
         // ---- round 2: row k+1 of W at the own slots
@@ -549,14 +552,17 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         }
 
         const T inv = recip(piv);
+        // corrections of the pairs j < i, in chunks of eight (entries past i
+        // are zero in b1, b2 and Ps, so a partial chunk adds nothing)
         T r1 = zero<T>(), rl = zero<T>(), rk = zero<T>();
-#pragma unroll 4
-        for (int j = 0; j < i; ++j) {
-            const T uj = Ps[j * BC_T + t], vj = Ps[(NB + j) * BC_T + t];
-            const T b1u = b1[j], b1v = b1[NB + j];
-            r1 = r1 + b1u * vj - b1v * uj;
-            if (has_next) {
-                const T b2u = b2[j], b2v = b2[NB + j];
+        for (int j0 = 0; j0 < i; j0 += 8) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int j = j0 + q;
+                const T uj = Ps[2 * (j * BC_T + t)], vj = Ps[2 * (j * BC_T + t) + 1];
+                const T b1u = b1[2 * j], b1v = b1[2 * j + 1];
+                const T b2u = b2[2 * j], b2v = b2[2 * j + 1];
+                r1 = r1 + b1u * vj - b1v * uj;
                 rl = rl + b2u * vj - b2v * uj;
                 rk = rk + b1u * b2v - b1v * b2u;
             }
@@ -564,28 +570,28 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         T u = zero<T>();
         if (act1) {
             u = (w1 + r1) * inv;
-            Ps[i * BC_T + t] = u;
             PU[(int64_t)i * n + x] = u;
         }
+        Ps[2 * (i * BC_T + t)] = u;
         if (!has_next) break;
 
         // ---- row k+2 (the next pivot row) at the own slots -> V_{i+1}
         const T u_kl = (w12 + rk) * inv;   // U_i at row k+2
-        const T v_kl = b2[NB + i];         // V_i at row k+2
+        const T v_kl = b2[2 * i + 1];      // V_i at row k+2
         double best = -1.0;
         T r = zero<T>();
         if (own && lg >= k + 3) {
-            r = w2 + rl + u_kl * Ps[(NB + i) * BC_T + t] - v_kl * u;
-            Ps[(NB + i + 1) * BC_T + t] = r;
+            r = w2 + rl + u_kl * Ps[2 * (i * BC_T + t) + 1] - v_kl * u;
             PV[(int64_t)(i + 1) * n + x] = r;
             const double m = (double)mag2(r);
             if (m == m) best = m;
         }
+        Ps[2 * ((i + 1) * BC_T + t) + 1] = r;
         const int npar = (i + 1) & 1;
         bc_publish<T, NB>(best, lg, r, lg, x, own, i + 1, &co.part[npar * BC_MAXG + blockIdx.x],
                           co.cand + ((int64_t)npar * BC_MAXG + blockIdx.x) * 2 * NB, Ps, r_best, r_idx, &r_t);
         ++nbar;
-        bc_barrier(co.bar, bar_base + nbar * G, &st->err);
+        bc_barrier(co.flags, epoch0 + nbar, &st->err);
     }
 
     if (own) a.perm[lg] = x;   // logical -> slot, for the trailing update
@@ -772,7 +778,7 @@ static BPLayout<T> bp_layout(int64_t n, int nb) {
     L.st = o; o = al(o + sizeof(BPState<T>));
     L.part = o; o = al(o + sizeof(BPPart<T>) * 2 * BC_MAXG);
     L.cand = o; o = al(o + sizeof(T) * 2 * BC_MAXG * 2 * nb);
-    L.bar = o; o = al(o + sizeof(int));
+    L.bar = o; o = al(o + sizeof(int) * BC_MAXG);
     L.total = o;
     return L;
 }
@@ -800,7 +806,7 @@ static bool coop_enabled() {
 }
 
 template <typename T, int NB>
-static hipError_t launch_panel(const BPArgs<T>& a, const BPCoop<T>& co, int k0, int np, int bar_base, int G,
+static hipError_t launch_panel(const BPArgs<T>& a, const BPCoop<T>& co, int k0, int np, int epoch0, int G,
                                hipStream_t s) {
     const size_t smem = sizeof(T) * 2 * NB * BC_T;
     static bool attr = false;
@@ -812,7 +818,7 @@ static hipError_t launch_panel(const BPArgs<T>& a, const BPCoop<T>& co, int k0, 
     }
     BPArgs<T> a_ = a;
     BPCoop<T> co_ = co;
-    void* args[] = {&a_, &co_, &k0, &np, &bar_base};
+    void* args[] = {&a_, &co_, &k0, &np, &epoch0};
     return hipLaunchCooperativeKernel((const void*)bp_panel<T, NB>, dim3(G), dim3(BC_T), args, (unsigned)smem, s);
 }
 
@@ -848,14 +854,14 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
     BPCoop<T> co{};
     co.part = (BPPart<T>*)(base + L.part);
     co.cand = (T*)(base + L.cand);
-    co.bar = (int*)(base + L.bar);
-    const bool coop = coop_enabled() && (nb == 16 || nb == 32);
+    co.flags = (int*)(base + L.bar);
+    const bool coop = coop_enabled() && nb == (std::is_same<T, c128>::value ? 16 : 32);
     hipError_t e = hipMemcpyAsync(W[0], A, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return (int)e;
     a.W = W[0];
-    hipLaunchKernelGGL(bp_init<T>, dim3(64), dim3(256), 0, s, a, co.bar);
+    hipLaunchKernelGGL(bp_init<T>, dim3(64), dim3(256), 0, s, a, co.flags);
     int cur = 0;
-    int bar_base = 0;
+    int epoch = 0;   // grid barriers passed so far (cooperative panels)
     auto blocks = [&](int64_t c_lo) { return (unsigned)((n - c_lo + BP_THREADS - 1) / BP_THREADS); };
     for (int64_t k0 = 0; k0 + 1 < n; k0 += 2 * nb) {
         const int np = (int)((n - k0) / 2 < nb ? (n - k0) / 2 : nb);
@@ -863,10 +869,10 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
         const int G = (int)((n - k0 + BC_T - 1) / BC_T);
         const bool panel_coop = coop && G <= BC_MAXG;
         if (panel_coop) {
-            e = (nb == 32) ? launch_panel<T, 32>(a, co, (int)k0, np, bar_base, G, s)
-                           : launch_panel<T, 16>(a, co, (int)k0, np, bar_base, G, s);
+            if constexpr (std::is_same<T, c128>::value) e = launch_panel<T, 16>(a, co, (int)k0, np, epoch, G, s);
+            else e = launch_panel<T, 32>(a, co, (int)k0, np, epoch, G, s);
             if (e != hipSuccess) return (int)e;
-            bar_base += np * G;
+            epoch += np;   // one barrier for the first row, one per pair step but the last
         } else {
         // first pivot row of the panel
         hipLaunchKernelGGL((bp_row<T, false, true>), dim3(blocks(k0 + 1)), dim3(BP_THREADS), 0, s, a, (int)k0, 0,

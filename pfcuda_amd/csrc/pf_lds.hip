@@ -34,6 +34,7 @@
 
 #include "pf_batch.h"
 #include "pf_types.h"
+#include "pf_wave.h"
 
 namespace pfh {
 
@@ -46,13 +47,6 @@ typedef __attribute__((address_space(3))) int lds_i32;
 __device__ __forceinline__ int row_off(int i, int n16) { return i * (n16 - 2) - (i * (i - 1)) / 2 - 1; }
 
 __device__ __forceinline__ int swap_pos(int c, int p, int q) { return c == p ? q : (c == q ? p : c); }
-
-__device__ __forceinline__ double readlane_f64(double x, int l) {
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
-    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
 
 // value held for position p (lane p % 64, register p / 64); p wave-uniform.
 // Every register is read out and the choice is made on scalars, so the
@@ -67,38 +61,6 @@ __device__ __forceinline__ double pos_value(const double (&v)[NM], int p) {
         x = (m == i) ? xi : x;
     }
     return x;
-}
-
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ double dpp_f64(double v) {
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)b, (int)(unsigned)b, CTRL, ROW_MASK, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(b >> 32), (int)(unsigned)(b >> 32), CTRL, ROW_MASK, 0xF, false);
-    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-}
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ int dpp_i32(int v) {
-    return __builtin_amdgcn_update_dpp(v, v, CTRL, ROW_MASK, 0xF, false);
-}
-
-// wave maximum (inputs >= -1, no NaN), result in every lane via lane 63
-__device__ __forceinline__ double wave_max(double m) {
-    m = fmax(m, dpp_f64<0xB1, 0xF>(m));    // quad_perm [1,0,3,2]
-    m = fmax(m, dpp_f64<0x4E, 0xF>(m));    // quad_perm [2,3,0,1]
-    m = fmax(m, dpp_f64<0x141, 0xF>(m));   // row_half_mirror
-    m = fmax(m, dpp_f64<0x140, 0xF>(m));   // row_mirror
-    m = fmax(m, dpp_f64<0x142, 0xA>(m));   // row_bcast:15
-    m = fmax(m, dpp_f64<0x143, 0xC>(m));   // row_bcast:31
-    return readlane_f64(m, 63);
-}
-__device__ __forceinline__ int wave_min(int k) {
-    k = min(k, dpp_i32<0xB1, 0xF>(k));
-    k = min(k, dpp_i32<0x4E, 0xF>(k));
-    k = min(k, dpp_i32<0x141, 0xF>(k));
-    k = min(k, dpp_i32<0x140, 0xF>(k));
-    k = min(k, dpp_i32<0x142, 0xA>(k));
-    k = min(k, dpp_i32<0x143, 0xC>(k));
-    return __builtin_amdgcn_readlane(k, 63);
 }
 
 // pivot search: among candidates (key in (lo_key, n), magnitude not NaN) the

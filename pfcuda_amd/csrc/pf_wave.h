@@ -1,0 +1,52 @@
+// Wave64 helpers shared by the kernels: DPP reductions and lane reads.
+//
+// The reductions run in six DPP steps (quad_perm [1,0,3,2], [2,3,0,1],
+// row_half_mirror, row_mirror, row_bcast:15, row_bcast:31) and read the
+// result from lane 63, so every lane gets the same value.  All lanes of the
+// wave must be active.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pfh {
+
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)b, (int)(unsigned)b, CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(b >> 32), (int)(unsigned)(b >> 32), CTRL, ROW_MASK, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_i32(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, ROW_MASK, 0xF, false);
+}
+
+// wave maximum (no NaN inputs), in every lane
+__device__ __forceinline__ double wave_max(double m) {
+    m = fmax(m, dpp_f64<0xB1, 0xF>(m));
+    m = fmax(m, dpp_f64<0x4E, 0xF>(m));
+    m = fmax(m, dpp_f64<0x141, 0xF>(m));
+    m = fmax(m, dpp_f64<0x140, 0xF>(m));
+    m = fmax(m, dpp_f64<0x142, 0xA>(m));
+    m = fmax(m, dpp_f64<0x143, 0xC>(m));
+    return readlane_f64(m, 63);
+}
+// wave minimum, in every lane
+__device__ __forceinline__ int wave_min(int k) {
+    k = min(k, dpp_i32<0xB1, 0xF>(k));
+    k = min(k, dpp_i32<0x4E, 0xF>(k));
+    k = min(k, dpp_i32<0x141, 0xF>(k));
+    k = min(k, dpp_i32<0x140, 0xF>(k));
+    k = min(k, dpp_i32<0x142, 0xA>(k));
+    k = min(k, dpp_i32<0x143, 0xC>(k));
+    return __builtin_amdgcn_readlane(k, 63);
+}
+
+}  // namespace pfh
