@@ -264,6 +264,15 @@ def main():
                                  "sign_blocked": pb8.real, "sign_closed_form": sign_det_q,
                                  "note": "closed form log|Pf| = sum log s_i is exact only to ~kappa*u of the rounded Q"}
     print(f"# c3: {large_res['c3_f64_n8192']}", file=sys.stderr, flush=True)
+    # mixed vs FP64 on the same matrix: MIXED runs the FP64 blocked path (the
+    # trailing update stores and streams FP64 W at 8 flop/B, under the FP64
+    # ridge of 9.8 flop/B, so a lower-precision product would not shorten it)
+    tm8, lm8, pm8 = time_slog(A8, _lib.PFH_METHOD_MIXED, 2)
+    large_res["mixed_vs_fp64_n8192"] = {"mixed_ms": tm8 * 1e3, "fp64_blocked_ms": tb8 * 1e3,
+                                        "dlog": abs(lm8 - lb8), "sign_agree": abs(pm8 - pb8) < 1e-12,
+                                        "within_1e-9": abs(lm8 - lb8) <= 1e-9,
+                                        "note": "MIXED executes the FP64 blocked path"}
+    print(f"# mixed: {large_res['mixed_vs_fp64_n8192']}", file=sys.stderr, flush=True)
     del A8
 
     # C5: gradient 1/2 A^-T through the HIP block sweep

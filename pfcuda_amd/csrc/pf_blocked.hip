@@ -525,8 +525,8 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         }
         const bool sw = p != k + 1;
         if (keeper) accumulate(lm, le, ph_re, ph_im, piv, sw);
-        // ---- interchange k+1 <-> p
-        __syncthreads();                                   // every thread has read Qpan
+        // ---- interchange k+1 <-> p (every read of Qpan in this step came
+        //      before the block barriers above)
         if (sw && t == 0) {
             Qpan[k + 1 - k0] = sp;
             if (p - k0 < 2 * NB + 2) Qpan[p - k0] = sa;
@@ -744,6 +744,62 @@ __global__ void __launch_bounds__(256) bp_trailing_mfma(const double* __restrict
     }
 }
 
+// complex128 form of bp_trailing_mfma: the same tiles and lane layout, with
+// each complex product as four real ones,
+//   Re C += Re A Re B - Im A Im B,   Im C += Re A Im B + Im A Re B.
+template <int NB>
+__global__ void __launch_bounds__(256) bp_trailing_mfma_c(const c128* __restrict__ Wo, c128* __restrict__ Wn,
+                                                          const c128* __restrict__ P, const int* __restrict__ perm,
+                                                          int64_t n, int lo, int np, int ntile, int pslot) {
+    constexpr int K = 2 * NB;
+    int tb = blockIdx.x, ti = 0;
+    while (tb >= ntile - ti) { tb -= ntile - ti; ++ti; }
+    const int tj = ti + tb;
+    const int r0 = lo + ti * BP_TILE + 16 * (threadIdx.x >> 6), c0 = lo + tj * BP_TILE;
+    const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+    const int n1 = (int)n - 1;
+    int pr[4], pc[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) pr[e] = perm[min(r0 + lq + 4 * e, n1)];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pc[u] = perm[min(c0 + 16 * u + lr, n1)];
+    const int pa = perm[min(r0 + lr, n1)];
+    const int sa = pslot ? pa : min(r0 + lr, n1);
+    int sb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) sb[u] = pslot ? pc[u] : min(c0 + 16 * u + lr, n1);
+    bt_d4 are[4], aim[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { are[u] = bt_d4{0.0, 0.0, 0.0, 0.0}; aim[u] = bt_d4{0.0, 0.0, 0.0, 0.0}; }
+#pragma unroll
+    for (int g = 0; g < K / 4; ++g) {
+        const int q = 4 * g + lq;
+        const bool on = (q % NB) < np;
+        const int qb = q < NB ? NB + q : q - NB;
+        const double sg = q < NB ? 1.0 : -1.0;
+        const c128 va = P[(int64_t)q * n + sa];
+        const double ar = on ? va.re : 0.0, ai = on ? va.im : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const c128 vb = P[(int64_t)qb * n + sb[u]];
+            const double br = on ? sg * vb.re : 0.0, bi = on ? sg * vb.im : 0.0;
+            are[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, are[u], 0, 0, 0);
+            are[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, are[u], 0, 0, 0);
+            aim[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, aim[u], 0, 0, 0);
+            aim[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, aim[u], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = r0 + lq + 4 * e, c = c0 + 16 * u + lr;
+            const c128 w = w_at(Wo, n, pr[e], pc[u]);   // clamped indices are valid
+            if (c < n && c > r) Wn[(int64_t)r * n + c] = c128{w.re + are[u][e], w.im + aim[u][e]};
+        }
+    }
+}
+
 template <typename T>
 __global__ void bp_finish(const BPState<T>* st, double* log_out, T* phase_out) {
     if (threadIdx.x != 0) return;
@@ -907,11 +963,22 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
                     hipLaunchKernelGGL(bp_trailing<T>, dim3(tiles), dim3(256), smem, s, W[cur], W[cur ^ 1], a.P,
                                        a.perm, n, (int)lo, nb, np, ntile, panel_coop ? 1 : 0);
                 }
+            } else if constexpr (std::is_same<T, c128>::value) {
+                if (nb == 16) {
+                    hipLaunchKernelGGL(bp_trailing_mfma_c<16>, dim3(tiles), dim3(256), 0, s, W[cur], W[cur ^ 1],
+                                       a.P, a.perm, n, (int)lo, np, ntile, panel_coop ? 1 : 0);
+                } else {
+                    hipLaunchKernelGGL(bp_trailing<T>, dim3(tiles), dim3(256), smem, s, W[cur], W[cur ^ 1], a.P,
+                                       a.perm, n, (int)lo, nb, np, ntile, panel_coop ? 1 : 0);
+                }
             } else {
                 hipLaunchKernelGGL(bp_trailing<T>, dim3(tiles), dim3(256), smem, s, W[cur], W[cur ^ 1], a.P, a.perm,
                                    n, (int)lo, nb, np, ntile, panel_coop ? 1 : 0);
             }
-            hipLaunchKernelGGL(bp_reset_perm<T>, dim3(64), dim3(256), 0, s, a.perm, lo, n);
+            // the per-step row kernels start a panel from the identity map; the
+            // cooperative panel builds its own (the next panel is cooperative
+            // whenever this one was: the panel only shrinks)
+            if (!panel_coop) hipLaunchKernelGGL(bp_reset_perm<T>, dim3(64), dim3(256), 0, s, a.perm, lo, n);
             cur ^= 1;
         }
     }
