@@ -1,0 +1,319 @@
+// Batched blocked Parlett-Reid, FP64 slog: one workgroup per matrix, with
+// several workgroups resident per CU.
+//
+// The LDS-resident kernel (pf_lds.hip) keeps the whole triangle in LDS, so
+// above n = 128 one matrix occupies a CU and its serial pair-step chain is
+// exposed.  Here the matrix lives in global memory (the input for the first
+// panel, then two per-workgroup slabs that alternate, both L2 / MALL
+// traffic) and only the panel vectors live in LDS (2 nb n doubles), so
+// several matrices share a CU and hide each other's latency:
+//
+//   panel (nb pair steps, left-looking inside the panel): thread t owns
+//     column ("slot") k0 + t of the panel's starting matrix and keeps its
+//     U_j, V_j in LDS.  Pair step i (k = k0 + 2i) reads row k+1 of the stale
+//     matrix at the own slot and the pair's corrections
+//       U_i[x] = (W[y1][x] + sum_{j<i} U_j[y1] V_j[x] - V_j[y1] U_j[x]) / a
+//     and forms the next pivot row the same way (V_{i+1} = row k+2); the
+//     pivot is a workgroup argmax (DPP per wave, then across the waves).
+//     An interchange only changes maps: logical -> slot for the panel's own
+//     positions (Qpan, LDS, double-buffered by step parity) and slot ->
+//     logical per thread (a register).
+//   trailing update (once per panel): C = stale[Q r][Q c] + A B with
+//     A[r][2j] = U_j, A[r][2j+1] = V_j, B[2j][c] = V_j, B[2j+1][c] = -U_j
+//     (the two terms of a pair adjacent in K), one v_mfma_f64_16x16x4f64
+//     per 16 x 16 x 4 step, operands from LDS, written compacted into the
+//     other slab.
+//
+// The arithmetic is the skew Schur complement S = A22 + (y x^T - x y^T)/a of
+// the reference's pf_sm_calc (src/pfaffian_sm.cu:71-89) and the lazy row
+// update of its large driver (src/slog_pfaffian_lg.cu:75-112), a pair step
+// at a time with the pivot search of src/pfaffian_cpu.cpp:47-52 (largest
+// magnitude, lowest index on ties, NaN never wins).
+#include <cstdlib>
+
+#include "pf_batch.h"
+#include "pf_types.h"
+#include "pf_wave.h"
+
+namespace pfh {
+
+namespace {
+
+typedef double bat_d4 __attribute__((ext_vector_type(4)));
+
+// Entry (r, c) of a matrix whose upper triangle is stored with row stride
+// ld; the load is unconditional (the diagonal when r == c).
+__device__ __forceinline__ double bat_w(const double* W, int ld, int r, int c) {
+    const int a = r < c ? r : c, b = r < c ? c : r;
+    const double v = W[(int64_t)a * ld + b];
+    return r < c ? v : (r > c ? -v : 0.0);
+}
+
+// workgroup argmax, part 1: the wave's best (|value|, key) into LDS
+__device__ __forceinline__ void bat_wave_best(double best, int key, double* r_best, int* r_key) {
+    const double b = wave_max(best);                                            // inputs >= -1, no NaN
+    const int k = wave_min((best == b && best >= 0.0) ? key : 0x7fffffff);      // ties: lowest key
+    if ((threadIdx.x & 63) == 0) { r_best[threadIdx.x >> 6] = b; r_key[threadIdx.x >> 6] = k; }
+}
+
+}  // namespace
+
+template <int NP, int NB>
+__global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
+    constexpr int NW = NP / 64;
+    extern __shared__ double bat_lds[];     // U_j, V_j of local slot s at [2 (j NP + s)] and [2 (j NP + s) + 1]
+    double* Pp = bat_lds;
+    __shared__ int Qpan[2][2 * NB + 2];     // slot of logical k0 + q, by pair-step parity
+    __shared__ int Qs[NP];                  // trailing update: local slot of logical lo + r
+    __shared__ double r_best[2][NW];        // per-wave argmax, by pair-step parity
+    __shared__ int r_key[2][NW];
+
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    double* slab0 = d.bslab + (int64_t)blockIdx.x * 2 * NP * NP;   // this workgroup's slab pair
+    double* slab1 = slab0 + (int64_t)NP * NP;
+
+    int64_t count = d.count;
+    if (d.count_dev) {
+        const int64_t cd = *d.count_dev;
+        count = cd < count ? cd : count;
+    }
+    for (int64_t jm = blockIdx.x; jm < count; jm += gridDim.x) {
+        const int idx = d.order ? d.order[(d.begin_dev ? *d.begin_dev : 0) + jm] : (int)jm;
+        const int n = __builtin_amdgcn_readfirstlane(d.sizes ? d.sizes[idx] : d.dense_n);
+        const int64_t off = d.offsets ? d.offsets[idx] : (int64_t)idx * n * n;
+        if (n < 0 || n > NP) {
+            if (t == 0) { d.log_out[idx] = __builtin_nan(""); ((double*)d.phase_out)[idx] = __builtin_nan(""); }
+            continue;
+        }
+        if (n == 0 || (n & 1)) {
+            if (t == 0) { d.log_out[idx] = -INFINITY; ((double*)d.phase_out)[idx] = 0.0; }
+            continue;
+        }
+        const double* src = (const double*)d.data + off;
+        int ld = n;
+        double* dst = slab0;
+        LogAcc la;
+        Phase<double> ph;
+        bool dead = false;
+
+        for (int k0 = 0; k0 < n; k0 += 2 * NB) {
+            const int np = min(NB, (n - k0) / 2);
+            const int s = t;                 // local slot
+            const int x = k0 + t;            // slot: index in src
+            const bool own = x < n;
+            const int xs = own ? x : k0;     // a valid slot for masked reads
+            int lg = x;                      // logical index of the own slot
+            __syncthreads();                 // the previous panel's LDS reads are done
+            for (int q = t; q < 2 * NB + 2; q += NP) Qpan[0][q] = k0 + q;
+#pragma unroll 4
+            for (int j = 0; j < NB; ++j) { Pp[2 * (j * NP + s)] = 0.0; Pp[2 * (j * NP + s) + 1] = 0.0; }
+            // first pivot row: row k0 of src (nothing pending)
+            {
+                double best = -1.0;
+                const double r0v = bat_w(src, ld, k0, xs);
+                const double r = (own && x > k0) ? r0v : 0.0;
+                Pp[2 * s + 1] = r;                                   // V_0
+                const double m = fabs(r);
+                if (own && x > k0 && m == m) best = m;
+                bat_wave_best(best, lg * 512 + s, r_best[0], r_key[0]);
+                __syncthreads();
+            }
+            for (int i = 0; i < np; ++i) {
+                const int k = k0 + 2 * i;
+                const int cb = i & 1;
+                const bool has_next = i + 1 < np;
+                double gb = r_best[cb][0];
+                int gk = r_key[cb][0];
+#pragma unroll
+                for (int u = 1; u < NW; ++u)
+                    if (r_best[cb][u] > gb || (r_best[cb][u] == gb && r_key[cb][u] < gk)) { gb = r_best[cb][u]; gk = r_key[cb][u]; }
+                const int sa = Qpan[cb][k + 1 - k0];
+                const int sb = has_next ? Qpan[cb][k + 2 - k0] : sa;
+                int p, sp;
+                if (gb >= 0.0) { p = gk >> 9; sp = k0 + (gk & 511); }
+                else { p = k + 1; sp = sa; }                         // no finite candidate (NaN row)
+                const double piv = Pp[2 * (i * NP + (sp - k0)) + 1]; // row k at column p
+                if (piv == 0.0) { dead = true; break; }              // the same decision in every thread
+                const bool sw = p != k + 1;
+                la.mul(fabs(piv));
+                ph.mul(piv);
+                if (sw) ph.flip();
+                // the next step's map: interchange k+1 <-> p
+                for (int q = t; q < 2 * NB + 2; q += NP) {
+                    int v = Qpan[cb][q];
+                    if (q == k + 1 - k0) v = sp;
+                    else if (sw && q == p - k0) v = sa;
+                    Qpan[cb ^ 1][q] = v;
+                }
+                if (x == sp) lg = k + 1;
+                else if (sw && x == sa) lg = p;
+                const int y1 = sp, y2 = (p == k + 2) ? sa : sb;
+                if (k + 2 >= n) break;                               // last pair of the matrix
+
+                const double w1 = bat_w(src, ld, y1, xs);
+                const double w2 = bat_w(src, ld, y2, xs);
+                const double w12 = bat_w(src, ld, y1, y2);
+                const int s1 = y1 - k0, s2 = y2 - k0;
+                double r1 = 0.0, rl = 0.0, rk = 0.0;
+                for (int j = 0; j < i; ++j) {
+                    const double uj = Pp[2 * (j * NP + s)], vj = Pp[2 * (j * NP + s) + 1];
+                    const double b1u = Pp[2 * (j * NP + s1)], b1v = Pp[2 * (j * NP + s1) + 1];
+                    const double b2u = Pp[2 * (j * NP + s2)], b2v = Pp[2 * (j * NP + s2) + 1];
+                    r1 = r1 + b1u * vj - b1v * uj;
+                    rl = rl + b2u * vj - b2v * uj;
+                    rk = rk + b1u * b2v - b1v * b2u;
+                }
+                const double inv = 1.0 / piv;
+                const double u = (own && lg >= k + 2) ? (w1 + r1) * inv : 0.0;
+                const double vi = Pp[2 * (i * NP + s) + 1];          // V_i at the own slot
+                Pp[2 * (i * NP + s)] = u;                            // U_i
+                if (!has_next) break;
+                const double u_kl = (w12 + rk) * inv;                // U_i at row k+2
+                const double v_kl = Pp[2 * (i * NP + s2) + 1];       // V_i at row k+2
+                double best = -1.0;
+                double r = 0.0;
+                if (own && lg >= k + 3) {
+                    r = w2 + rl + u_kl * vi - v_kl * u;
+                    const double m = fabs(r);
+                    if (m == m) best = m;
+                }
+                Pp[2 * ((i + 1) * NP + s) + 1] = r;                  // V_{i+1} = row k+2
+                bat_wave_best(best, lg * 512 + s, r_best[cb ^ 1], r_key[cb ^ 1]);
+                __syncthreads();
+            }
+            if (dead) break;
+            const int lo = k0 + 2 * np;
+            if (lo >= n) break;
+
+            // ---- trailing update of [lo, n) into dst, 16 x 16 upper tiles ----
+            if (own && lg >= lo) Qs[lg - lo] = s;
+            __syncthreads();
+            const int m = n - lo;
+            const int mt = (m + 15) >> 4;
+            const int ntile = mt * (mt + 1) / 2;
+            const int ng = (2 * np + 3) >> 2;                        // K steps of 4
+            for (int tt = wv; tt < ntile; tt += NW) {
+                int I = 0, rem = tt;
+                while (rem >= mt - I) { rem -= mt - I; ++I; }
+                const int J = I + rem;
+                // stale entries first (global), then the products (LDS operands)
+                double wv4[4];
+                int rr[4];
+                const int cc = 16 * J + lr;
+                const int scc = Qs[min(cc, m - 1)] + k0;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    rr[e] = 16 * I + lq + 4 * e;
+                    wv4[e] = bat_w(src, ld, Qs[min(rr[e], m - 1)] + k0, scc);
+                }
+                const int sA = Qs[min(16 * I + lr, m - 1)], sB = Qs[min(cc, m - 1)];
+                bat_d4 acc = {0.0, 0.0, 0.0, 0.0};
+                for (int g = 0; g < ng; ++g) {
+                    const int q = 4 * g + lq;                        // K index: pair q / 2
+                    const int j = q >> 1;
+                    const bool on = j < np;
+                    const int jj = on ? j : 0;
+                    const double a = Pp[2 * (jj * NP + sA) + (q & 1)];          // U_j, V_j at the row
+                    const double bb = Pp[2 * (jj * NP + sB) + ((q & 1) ^ 1)];   // V_j, U_j at the column
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(on ? a : 0.0, on ? ((q & 1) ? -bb : bb) : 0.0, acc,
+                                                               0, 0, 0);
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (rr[e] < m && cc < m && cc > rr[e]) dst[(int64_t)(lo + rr[e]) * NP + (lo + cc)] = wv4[e] + acc[e];
+            }
+            src = dst;
+            ld = NP;
+            dst = (dst == slab0) ? slab1 : slab0;
+        }
+        if (t == 0) {
+            d.log_out[idx] = dead ? -INFINITY : la.value();
+            ((double*)d.phase_out)[idx] = dead ? 0.0 : ph.s;
+        }
+    }
+}
+
+namespace {
+
+int bat_num_cu() {
+    static int v = 0;
+    if (v == 0) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    }
+    return v;
+}
+
+template <int NP, int NB>
+constexpr size_t bat_lds_bytes() { return sizeof(double) * 2 * NB * NP; }
+
+// resident grid (workgroups per CU x CUs); also sets the kernel's LDS limit,
+// which the occupancy query needs
+template <int NP, int NB>
+int bat_grid() {
+    static int g = 0;
+    if (g == 0) {
+        hipFuncSetAttribute((const void*)pf_bat_kernel<NP, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bat_lds_bytes<NP, NB>());
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, pf_bat_kernel<NP, NB>, NP, bat_lds_bytes<NP, NB>()) !=
+                hipSuccess || occ <= 0)
+            occ = 1;
+        g = occ * bat_num_cu();
+    }
+    return g;
+}
+
+template <int NP, int NB>
+int bat_launch(const BatchDesc& d, hipStream_t s) {
+    if (d.count <= 0) return 0;
+    auto kern = pf_bat_kernel<NP, NB>;
+    constexpr size_t bytes = bat_lds_bytes<NP, NB>();
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return (int)e;
+        attr = true;
+    }
+    int64_t grid = bat_grid<NP, NB>();
+    const int64_t per = 2 * (int64_t)NP * NP;   // one slab pair per workgroup
+    if (!d.bslab || d.bslab_elems < per) return (int)hipErrorInvalidValue;
+    if (grid > d.bslab_elems / per) grid = d.bslab_elems / per;
+    if (grid > d.count) grid = d.count;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NP), bytes, s, d);
+    return (int)hipGetLastError();
+}
+
+}  // namespace
+
+constexpr int BAT_NB = 16;
+
+namespace {
+template <int NP>
+int64_t bat_need(int64_t count) {
+    const int64_t g = bat_grid<NP, BAT_NB>();
+    return (count < g ? count : g) * 2 * NP * NP;
+}
+}  // namespace
+
+// doubles of workspace for batches of `count` matrices with n <= maxn: one
+// slab pair (2 NP^2 doubles) per resident workgroup of the widest
+// instantiation a bucket can need (the buckets run one after another)
+int64_t bat_ws_elems(int maxn, int64_t count) {
+    if (maxn <= 64 || count <= 0) return 0;
+    int64_t v = bat_need<128>(count);
+    if (maxn > 128) { const int64_t b = bat_need<192>(count); v = b > v ? b : v; }
+    if (maxn > 192) { const int64_t c = bat_need<256>(count); v = c > v ? c : v; }
+    return v;
+}
+
+int launch_bat_f64(const BatchDesc& d, int maxn, hipStream_t s) {
+    if (maxn <= 128) return bat_launch<128, BAT_NB>(d, s);
+    if (maxn <= 192) return bat_launch<192, BAT_NB>(d, s);
+    if (maxn <= 256) return bat_launch<256, BAT_NB>(d, s);
+    return (int)hipErrorInvalidValue;
+}
+
+}  // namespace pfh

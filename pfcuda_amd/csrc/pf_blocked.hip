@@ -670,9 +670,11 @@ __global__ void __launch_bounds__(256) bp_trailing(const T* __restrict__ Wo, T* 
 }
 
 // FP64 trailing update on the matrix cores: the same sum as bp_trailing, as
-// the product  C += A B  with  A[r][q] = P[q][s r]  (rows U_0..U_{nb-1},
-// V_0..V_{nb-1} of the panel store) and  B[q][c] = [V_0..V_{nb-1},
-// -U_0..-U_{nb-1}][s c]  (K = 2 nb), per 64 x 64 upper tile.  Wave w owns
+// the product  C += A B  (K = 2 nb) per 64 x 64 upper tile, with the two
+// terms of a pair adjacent in K (they nearly cancel, and summed apart the
+// partial sums grow and lose digits):
+//   A[r][2j] = U_j[s r],  A[r][2j+1] = V_j[s r],
+//   B[2j][c] = V_j[s c],  B[2j+1][c] = -U_j[s c].  Wave w owns
 // tile rows 16 w .. 16 w + 15 and four 16 x 16 accumulators across the
 // columns, fed by v_mfma_f64_16x16x4f64 (A operand: lane l holds
 // A[l % 16][l / 16], B operand B[l / 16][l % 16], accumulator register e holds
@@ -709,11 +711,13 @@ __global__ void __launch_bounds__(256) bp_trailing_mfma(const double* __restrict
     double av[K / 4], bv[K / 4][4];
 #pragma unroll
     for (int g = 0; g < K / 4; ++g) {
-        const int q = 4 * g + lq;                  // this lane's K index
-        const bool on = (q % NB) < np;
-        const int qb = q < NB ? NB + q : q - NB;   // B row: V_q, then U_{q - nb}
-        const double sg = q < NB ? 1.0 : -1.0;
-        const double va = P[(int64_t)q * n + sa];
+        const int q = 4 * g + lq;                  // this lane's K index: pair q / 2
+        const int j = q >> 1;
+        const bool on = j < np;
+        const int qa = (q & 1) ? NB + j : j;       // A row: U_j, V_j
+        const int qb = (q & 1) ? j : NB + j;       // B row: V_j, -U_j
+        const double sg = (q & 1) ? -1.0 : 1.0;
+        const double va = P[(int64_t)qa * n + sa];
         av[g] = on ? va : 0.0;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -744,8 +748,8 @@ __global__ void __launch_bounds__(256) bp_trailing_mfma(const double* __restrict
     }
 }
 
-// complex128 form of bp_trailing_mfma: the same tiles and lane layout, with
-// each complex product as four real ones,
+// complex128 form of bp_trailing_mfma: the same tiles, lane layout and K
+// order, with each complex product as four real ones,
 //   Re C += Re A Re B - Im A Im B,   Im C += Re A Im B + Im A Re B.
 template <int NB>
 __global__ void __launch_bounds__(256) bp_trailing_mfma_c(const c128* __restrict__ Wo, c128* __restrict__ Wn,
@@ -773,11 +777,13 @@ __global__ void __launch_bounds__(256) bp_trailing_mfma_c(const c128* __restrict
     for (int u = 0; u < 4; ++u) { are[u] = bt_d4{0.0, 0.0, 0.0, 0.0}; aim[u] = bt_d4{0.0, 0.0, 0.0, 0.0}; }
 #pragma unroll
     for (int g = 0; g < K / 4; ++g) {
-        const int q = 4 * g + lq;
-        const bool on = (q % NB) < np;
-        const int qb = q < NB ? NB + q : q - NB;
-        const double sg = q < NB ? 1.0 : -1.0;
-        const c128 va = P[(int64_t)q * n + sa];
+        const int q = 4 * g + lq;                  // pair q / 2
+        const int j = q >> 1;
+        const bool on = j < np;
+        const int qa = (q & 1) ? NB + j : j;
+        const int qb = (q & 1) ? j : NB + j;
+        const double sg = (q & 1) ? -1.0 : 1.0;
+        const c128 va = P[(int64_t)qa * n + sa];
         const double ar = on ? va.re : 0.0, ai = on ? va.im : 0.0;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {

@@ -150,7 +150,7 @@ int bucket_edges(int dt, int32_t* e) {
 }
 
 struct RaggedLayout {
-    size_t hist, cursor, order, begin, cnt, edges, spill, slabs, total;
+    size_t hist, cursor, order, begin, cnt, edges, spill, bat, slabs, total;
     int glb_grid;
     int64_t slab_elems;
 };
@@ -174,6 +174,9 @@ RaggedLayout ragged_layout(int dt, int64_t count, int32_t max_n) {
     L.spill = off;
     if (dt == PFH_F64 && max_n > lds_max_n())   // spill slabs of the LDS kernel's 192 < n <= 256 bucket
         off = align_up(off + sizeof(double) * (size_t)lds_spill_elems() * lds_spill_slabs());
+    L.bat = off;
+    if (dt == PFH_F64 && max_n > 64)            // slab pairs of the batched blocked kernel
+        off = align_up(off + sizeof(double) * (size_t)bat_ws_elems(max_n, count));
     L.slabs = off;
     L.glb_grid = 0;
     L.slab_elems = 0;
@@ -230,6 +233,10 @@ int ragged_typed(int dt, const void* data, const int64_t* offsets, const int32_t
             d.slab = (double*)(ws + L.spill);
             d.slab_elems = lds_spill_elems();
         }
+        if (dt == PFH_F64 && max_n > 64) {
+            d.bslab = (double*)(ws + L.bat);
+            d.bslab_elems = bat_ws_elems(max_n, count);
+        }
         if (b >= nb) {
             err = launch_glb<T>(d, (T*)(ws + L.slabs), L.slab_elems, L.glb_grid, s);
         } else {
@@ -256,13 +263,21 @@ int pfaffian_dense(int dt, const void* A, int64_t batch, int32_t n, void* out, h
     return launch_reg(dt, MODE_PROD, d, n < 2 ? 2 : n, s);
 }
 
+// Workspace of an f64 slog of a dense batch at n <= 256: the batched
+// blocked kernel's slab pairs (n > 64), then the LDS kernel's spill slabs
+// (n > 192).
+size_t dense_bat_bytes(int dt, int64_t batch, int32_t n) {
+    if (dt != PFH_F64 || n <= 64 || n > reg_max_n(dt)) return 0;
+    return align_up(sizeof(double) * (size_t)bat_ws_elems(n, batch));
+}
 size_t dense_spill_bytes(int dt, int32_t n) {
     if (dt != PFH_F64 || n <= lds_max_n() || n > reg_max_n(dt)) return 0;
     return sizeof(double) * (size_t)lds_spill_elems() * lds_spill_slabs();
 }
+size_t dense_ws_bytes(int dt, int64_t batch, int32_t n) { return dense_bat_bytes(dt, batch, n) + dense_spill_bytes(dt, n); }
 
 size_t slog_ws_bytes(int dt, int64_t batch, int32_t n, int method) {
-    if (n <= reg_max_n(dt) && method == PFH_METHOD_AUTO) return dense_spill_bytes(dt, n);
+    if (n <= reg_max_n(dt) && method == PFH_METHOD_AUTO) return dense_ws_bytes(dt, batch, n);
     const size_t es = dtype_bytes(dt);
     if (method == PFH_METHOD_BLOCKED || method == PFH_METHOD_MIXED ||
         (method == PFH_METHOD_AUTO && n > reg_max_n(dt)))
@@ -280,13 +295,20 @@ int slog_dense(int dt, const T* A, int64_t batch, int32_t n, int method, double*
     if (method == PFH_METHOD_AUTO && n <= reg_max_n(dt)) {
         BatchDesc d{};
         d.data = A; d.count = batch; d.dense_n = n; d.log_out = log_out; d.phase_out = phase_out;
-        const size_t sb = dense_spill_bytes(dt, n);
+        const size_t sb = dense_ws_bytes(dt, batch, n);
         if (sb) {
             void* ws = nullptr;
             int err = resolve_ws(ws_in, ws_bytes, sb, &ws);
             if (err) return err;
-            d.slab = (double*)ws;
-            d.slab_elems = lds_spill_elems();
+            const size_t bb = dense_bat_bytes(dt, batch, n);
+            if (bb) {
+                d.bslab = (double*)ws;
+                d.bslab_elems = bat_ws_elems(n, batch);
+            }
+            if (dense_spill_bytes(dt, n)) {
+                d.slab = (double*)((char*)ws + bb);
+                d.slab_elems = lds_spill_elems();
+            }
         }
         return launch_reg(dt, MODE_SLOG, d, n < 2 ? 2 : n, s);
     }

@@ -359,27 +359,34 @@ int reg_grad_max_n(int dtype) {
 }
 
 // f64 kernel choice for slog and the product form: "lanes" (column per lane,
-// above) or "lds" (matrix in LDS, MFMA tile updates, pf_lds.hip; above
-// n = 192 only when the caller supplied spill slabs).  Default, from
-// per-bucket timings of the 16384-matrix ragged batch on MI355X: slog takes
-// lds for n > 64 and lanes up to 64; the product form keeps lanes, whose
-// rank-2 updates round as the reference's (rel 1e-12 on 32 x 32).
-// PFH_F64_KERNEL=lanes|lds forces one where it applies.
+// above), "lds" (matrix in LDS, MFMA tile updates, pf_lds.hip; above
+// n = 192 only when the caller supplied spill slabs) or "bat" (batched
+// blocked, matrix in global slabs, several matrices per CU, pf_bat.hip; only
+// when the caller supplied its slabs).  Default, from per-bucket timings of
+// the 16384-matrix ragged batch on MI355X (ms, lds | bat: 64-128 0.95 | 0.61,
+// 128-192 3.22 | 1.46, 192-256 8.32 | 3.31): slog takes bat for n > 64 when
+// its slabs are there, else lds, and lanes up to 64; the product form keeps
+// lanes, whose rank-2 updates round as the reference's (rel 1e-12 on
+// 32 x 32).  PFH_F64_KERNEL=lanes|lds|bat forces one where it applies.
 static int f64_kernel_override() {
     static int v = -2;
     if (v == -2) {
         const char* e = getenv("PFH_F64_KERNEL");
-        v = !e ? -1 : strcmp(e, "lanes") == 0 ? 0 : strcmp(e, "lds") == 0 ? 2 : -1;
+        v = !e ? -1 : strcmp(e, "lanes") == 0 ? 0 : strcmp(e, "lds") == 0 ? 2 : strcmp(e, "bat") == 0 ? 3 : -1;
     }
     return v;
 }
 
-// 0 lanes, 2 lds
+// 0 lanes, 2 lds, 3 bat (batched blocked, pf_bat.hip; needs d.bslab)
 static int f64_kernel(int mode, int maxn, const BatchDesc& d) {
     if (mode == MODE_GRAD) return 0;
     if (mode == MODE_PROD && maxn > 64) return 0;
-    if (maxn > 256 || (maxn > lds_max_n() && !d.slab)) return 0;
+    if (maxn > 256) return 0;
     const int o = f64_kernel_override();
+    const bool bat_ok = mode == MODE_SLOG && maxn > 64 && d.bslab;
+    if (o == 3) return bat_ok ? 3 : 0;
+    if (o < 0 && bat_ok) return 3;
+    if (maxn > lds_max_n() && !d.slab) return 0;
     if (o >= 0) return o;
     if (mode == MODE_PROD) return 0;
     return maxn > 64 ? 2 : 0;
@@ -387,7 +394,9 @@ static int f64_kernel(int mode, int maxn, const BatchDesc& d) {
 
 int launch_reg(int dtype, int mode, const BatchDesc& d, int maxn, hipStream_t s) {
     if (dtype == DT_F64) {
-        if (f64_kernel(mode, maxn, d) == 2) return launch_lds_f64(mode, d, maxn, s);
+        const int kf = f64_kernel(mode, maxn, d);
+        if (kf == 2) return launch_lds_f64(mode, d, maxn, s);
+        if (kf == 3) return launch_bat_f64(d, maxn, s);
     }
     if (mode == MODE_GRAD) {
         switch (dtype) {

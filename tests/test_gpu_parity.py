@@ -201,6 +201,33 @@ def test_blocked_dies_mid_panel(pf, cuda, dtype):
     assert l0 == -math.inf and p0 == 0
 
 
+def test_batched_dies_at_first_and_mid_panel(pf, cuda):
+    # f64 slog at 64 < n <= 256 (the batched blocked kernel): an all-zero
+    # matrix dies at the first pair step; 50 zero rows/columns scattered in an
+    # n = 200 matrix leave 150 nonzero pivots, so it dies inside a panel;
+    # a healthy matrix in the same launch is unaffected.
+    rng = np.random.default_rng(91)
+    z = np.zeros((130, 130))
+    B = oracle.skew_gaussian(rng, 200)
+    B[150:, :] = 0
+    B[:, 150:] = 0
+    perm = rng.permutation(200)
+    B = np.ascontiguousarray(B[perm][:, perm])
+    C = oracle.skew_gaussian(rng, 256)
+    mats, sizes = [z, B, C], [130, 200, 256]
+    data = np.concatenate([m.reshape(-1) for m in mats])
+    offs = np.cumsum([0] + [n * n for n in sizes[:-1]])
+    gl, gp = pf.slog_pfaffian_ragged(torch.from_numpy(data).to(cuda), offs, sizes)
+    gl, gp = gl.cpu().numpy(), gp.cpu().numpy()
+    assert gl[0] == -math.inf and gp[0] == 0
+    assert gl[1] == -math.inf and gp[1] == 0
+    l0, p0 = oracle.slog(C)
+    assert abs(gl[2] - l0) < 1e-9 and gp[2] == p0
+    # the dense entry point on the same kind of batch
+    la, s = pf.slog_pfaffian(torch.from_numpy(np.stack([B, B])).to(cuda))
+    assert (la.cpu().numpy() == -math.inf).all() and (s.cpu().numpy() == 0).all()
+
+
 @pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
 @pytest.mark.parametrize("n", [8, 32, 64, 128, 200, 300])
 def test_grad_is_half_inverse_transpose(pf, cuda, dtype, n):
