@@ -35,6 +35,23 @@
 #include "pf_types.h"
 #include "pf_wave.h"
 
+// Pair steps per panel, per capacity: 2 nb NP doubles of LDS per workgroup
+// (fewer workgroups per CU as nb grows) against one trailing pass per nb
+// pairs.  Measured on the 16384-matrix batch (ms per bucket, nb 4 / 6 / 8 /
+// 12 / 16 / 32): 64-128 0.59 / 0.47 / 0.39 / 0.43 / 0.58 / 1.05, 128-192
+// 1.94 / 1.56 / 1.17 / 1.24 / 1.43 / 3.66, 192-256 5.03 / 3.56 / 2.76 / 2.60
+// / 3.30 / 5.94.  Capping the workgroups per CU below the occupancy limit
+// was slower at every width tried.
+#ifndef PFH_BAT_NB_128
+#define PFH_BAT_NB_128 8
+#endif
+#ifndef PFH_BAT_NB_192
+#define PFH_BAT_NB_192 8
+#endif
+#ifndef PFH_BAT_NB_256
+#define PFH_BAT_NB_256 12
+#endif
+
 namespace pfh {
 
 namespace {
@@ -192,24 +209,39 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
             const int m = n - lo;
             const int mt = (m + 15) >> 4;
             const int ntile = mt * (mt + 1) / 2;
-            const int ng = (2 * np + 3) >> 2;                        // K steps of 4
-            for (int tt = wv; tt < ntile; tt += NW) {
-                int I = 0, rem = tt;
+            // software-pipelined over this wave's tiles: the next tile's
+            // stale-entry gathers are issued before this tile's products
+            auto tile_ij = [&](int tt, int& I, int& J) {
+                int rem = tt;
+                I = 0;
                 while (rem >= mt - I) { rem -= mt - I; ++I; }
-                const int J = I + rem;
-                // stale entries first (global), then the products (LDS operands)
-                double wv4[4];
-                int rr[4];
-                const int cc = 16 * J + lr;
-                const int scc = Qs[min(cc, m - 1)] + k0;
+                J = I + rem;
+            };
+            auto gather = [&](int I, int J, double (&w4)[4]) {
+                const int scc = Qs[min(16 * J + lr, m - 1)] + k0;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    rr[e] = 16 * I + lq + 4 * e;
-                    wv4[e] = bat_w(src, ld, Qs[min(rr[e], m - 1)] + k0, scc);
+                for (int e = 0; e < 4; ++e) w4[e] = bat_w(src, ld, Qs[min(16 * I + lq + 4 * e, m - 1)] + k0, scc);
+            };
+            int I = 0, J = 0;
+            double wv4[4] = {0.0, 0.0, 0.0, 0.0};
+            if (wv < ntile) {
+                tile_ij(wv, I, J);
+                gather(I, J, wv4);
+            }
+            for (int tt = wv; tt < ntile; tt += NW) {
+                int In = I, Jn = J;
+                double wn[4] = {0.0, 0.0, 0.0, 0.0};
+                if (tt + NW < ntile) {
+                    tile_ij(tt + NW, In, Jn);
+                    gather(In, Jn, wn);
                 }
+                const int cc = 16 * J + lr;
                 const int sA = Qs[min(16 * I + lr, m - 1)], sB = Qs[min(cc, m - 1)];
                 bat_d4 acc = {0.0, 0.0, 0.0, 0.0};
-                for (int g = 0; g < ng; ++g) {
+                // unrolled to the panel width so that every operand read is
+                // issued ahead of the products (pairs past np are masked)
+#pragma unroll
+                for (int g = 0; g < (2 * NB + 3) / 4; ++g) {
                     const int q = 4 * g + lq;                        // K index: pair q / 2
                     const int j = q >> 1;
                     const bool on = j < np;
@@ -220,8 +252,14 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
                                                                0, 0, 0);
                 }
 #pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    if (rr[e] < m && cc < m && cc > rr[e]) dst[(int64_t)(lo + rr[e]) * NP + (lo + cc)] = wv4[e] + acc[e];
+                for (int e = 0; e < 4; ++e) {
+                    const int rr = 16 * I + lq + 4 * e;
+                    if (rr < m && cc < m && cc > rr) dst[(int64_t)(lo + rr) * NP + (lo + cc)] = wv4[e] + acc[e];
+                }
+                I = In;
+                J = Jn;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) wv4[e] = wn[e];
             }
             src = dst;
             ld = NP;
@@ -240,7 +278,7 @@ int bat_num_cu() {
     static int v = 0;
     if (v == 0) {
         int dev = 0;
-        hipGetDevice(&dev);
+        (void)hipGetDevice(&dev);
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
     }
     return v;
@@ -255,8 +293,8 @@ template <int NP, int NB>
 int bat_grid() {
     static int g = 0;
     if (g == 0) {
-        hipFuncSetAttribute((const void*)pf_bat_kernel<NP, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)bat_lds_bytes<NP, NB>());
+        (void)hipFuncSetAttribute((const void*)pf_bat_kernel<NP, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)bat_lds_bytes<NP, NB>());
         int occ = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, pf_bat_kernel<NP, NB>, NP, bat_lds_bytes<NP, NB>()) !=
                 hipSuccess || occ <= 0)
@@ -288,12 +326,11 @@ int bat_launch(const BatchDesc& d, hipStream_t s) {
 
 }  // namespace
 
-constexpr int BAT_NB = 16;
 
 namespace {
-template <int NP>
+template <int NP, int NB>
 int64_t bat_need(int64_t count) {
-    const int64_t g = bat_grid<NP, BAT_NB>();
+    const int64_t g = bat_grid<NP, NB>();
     return (count < g ? count : g) * 2 * NP * NP;
 }
 }  // namespace
@@ -303,16 +340,16 @@ int64_t bat_need(int64_t count) {
 // instantiation a bucket can need (the buckets run one after another)
 int64_t bat_ws_elems(int maxn, int64_t count) {
     if (maxn <= 64 || count <= 0) return 0;
-    int64_t v = bat_need<128>(count);
-    if (maxn > 128) { const int64_t b = bat_need<192>(count); v = b > v ? b : v; }
-    if (maxn > 192) { const int64_t c = bat_need<256>(count); v = c > v ? c : v; }
+    int64_t v = bat_need<128, PFH_BAT_NB_128>(count);
+    if (maxn > 128) { const int64_t b = bat_need<192, PFH_BAT_NB_192>(count); v = b > v ? b : v; }
+    if (maxn > 192) { const int64_t c = bat_need<256, PFH_BAT_NB_256>(count); v = c > v ? c : v; }
     return v;
 }
 
 int launch_bat_f64(const BatchDesc& d, int maxn, hipStream_t s) {
-    if (maxn <= 128) return bat_launch<128, BAT_NB>(d, s);
-    if (maxn <= 192) return bat_launch<192, BAT_NB>(d, s);
-    if (maxn <= 256) return bat_launch<256, BAT_NB>(d, s);
+    if (maxn <= 128) return bat_launch<128, PFH_BAT_NB_128>(d, s);
+    if (maxn <= 192) return bat_launch<192, PFH_BAT_NB_192>(d, s);
+    if (maxn <= 256) return bat_launch<256, PFH_BAT_NB_256>(d, s);
     return (int)hipErrorInvalidValue;
 }
 
