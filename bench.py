@@ -170,7 +170,7 @@ def main():
     flops = float(np.sum(sizes.astype(np.float64) ** 3) / 3.0)
     tri_bytes = float(np.sum(sizes.astype(np.float64) * (sizes - 1) / 2.0) * 8)
 
-    # dominant kernel: the largest bucket (192 < N <= 256), one pf_lds_kernel launch
+    # dominant kernel: the largest bucket (192 < N <= 256), one pf_bat_kernel launch
     # plus the few-microsecond bucketing kernels of the ragged call
     big = np.nonzero(sizes > 192)[0]
     sub = Ragged(torch, lib, data, offsets[big], sizes[big], _lib.PFH_F64)
@@ -335,8 +335,8 @@ def main():
                    "count": args.count, "seed": args.seed, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK,
-                     "traffic": committed_traffic("pf_lds_kernel<256,8,1>"),
-                     "kernel": "pf_lds_kernel<256,8,1> (bucket 192<N<=256)",
+                     "traffic": committed_traffic("pf_bat_kernel<256,12>"),
+                     "kernel": "pf_bat_kernel<256,12> (bucket 192<N<=256)",
                      "launch_ms": t_big * 1e3, "algorithmic_flop": big_flops},
         "whole_batch": {"flop": flops, "tflops": flops / (ms / 1e3) / 1e12,
                         "frac_fp64_peak": flops / (ms / 1e3) / FP64_PEAK, "triangle_bytes": tri_bytes},
