@@ -180,6 +180,65 @@ def main():
     big_flops = float(np.sum(sizes[big].astype(np.float64) ** 3) / 3.0)
     achieved = big_flops / t_big
 
+    # the plain all-FP64 GPU form, one matrix per call (the reference's
+    # per-matrix driver: PFH_METHOD_UNBLOCKED, three launches per pair step),
+    # timed on a sample of the batch and extrapolated to it
+    import ctypes
+    plain_res = None
+    k_pl = min(96, args.count)
+    pl_idx = np.random.default_rng(args.seed + 3).choice(args.count, size=k_pl, replace=False)
+    need = ctypes.c_size_t(0)
+    lib.pfh_slog_pfaffian_workspace_size(_lib.PFH_F64, 1, 256, _lib.PFH_METHOD_UNBLOCKED, need)
+    ws_pl = torch.empty(max(int(need.value), 1), dtype=torch.uint8, device=dev)
+    lo_pl = torch.empty(1, dtype=torch.float64, device=dev)
+    ph_pl = torch.empty(1, dtype=torch.float64, device=dev)
+
+    def plain_sample():
+        for i in pl_idx:
+            n_i = int(sizes[i])
+            st = lib.pfh_slog_pfaffian_method(_lib.PFH_F64, data.data_ptr() + 8 * int(offsets[i]), 1, n_i,
+                                              _lib.PFH_METHOD_UNBLOCKED, lo_pl.data_ptr(), ph_pl.data_ptr(),
+                                              ws_pl.data_ptr(), ws_pl.numel(), strm)
+            if st != 0:
+                raise RuntimeError(f"plain slog status {st}")
+    plain_sample()
+    torch.cuda.synchronize()
+    t_pl = time_events(torch, plain_sample, 1)
+    # extrapolate by the sample's share of the batch's pair steps
+    steps_all = float(np.sum(sizes // 2))
+    steps_pl = float(np.sum(sizes[pl_idx] // 2))
+    t_plain_batch = t_pl * steps_all / steps_pl
+    plain_res = {"sample_matrices": int(k_pl), "sample_ms": t_pl * 1e3,
+                 "extrapolated_batch_ms": t_plain_batch * 1e3, "pf_per_s": args.count / t_plain_batch,
+                 "speedup_of_batched": t_plain_batch / (ms / 1e3)}
+    print(f"# plain per-matrix: {plain_res}", file=sys.stderr, flush=True)
+
+    # the same batch shape in complex128
+    datac, offc, sizc = gen_ragged(torch, args.count, 16, 256, args.seed, torch.complex128, dev)
+    runc = Ragged(torch, lib, datac, offc, sizc, _lib.PFH_C128)
+    runc()
+    torch.cuda.synchronize()
+    t_c = time_events(torch, runc, 3)
+    c128_res = {"ms": t_c * 1e3, "pf_per_s": args.count / t_c,
+                "tflops": float(np.sum(4.0 * sizc.astype(np.float64) ** 3 / 3.0)) / t_c / 1e12}
+    if not args.no_cpu:
+        # a sample gathered on the device and checked against the FP64 oracle
+        import oracle
+        kc = min(256, args.count)
+        idc = np.random.default_rng(args.seed + 4).choice(args.count, size=kc, replace=False)
+        parts = [datac[int(offc[i]):int(offc[i]) + int(sizc[i]) ** 2] for i in idc]
+        hostc = torch.cat(parts).cpu().numpy()
+        offs_c = np.cumsum([0] + [int(sizc[i]) ** 2 for i in idc[:-1]]).astype(np.int64)
+        wl, wp = oracle.slog_ragged(hostc, offs_c, sizc[idc], threads=max(1, min(16, os.cpu_count() or 1)))
+        gl = runc.log.cpu().numpy()[idc]
+        gp = runc.ph.cpu().numpy()[idc]
+        c128_res["max_abs_dlog"] = float(np.max(np.abs(gl - wl)))
+        c128_res["max_abs_dphase"] = float(np.max(np.abs(gp - wp)))
+        c128_res["checked"] = int(kc)
+        del hostc, parts
+    print(f"# c128 ragged: {c128_res}", file=sys.stderr, flush=True)
+    del datac, runc
+
     # small batched product path: 1024 x 32x32 f64
     g = torch.Generator(device=dev)
     g.manual_seed(args.seed + 1)
@@ -340,6 +399,8 @@ def main():
                      "launch_ms": t_big * 1e3, "algorithmic_flop": big_flops},
         "whole_batch": {"flop": flops, "tflops": flops / (ms / 1e3) / 1e12,
                         "frac_fp64_peak": flops / (ms / 1e3) / FP64_PEAK, "triangle_bytes": tri_bytes},
+        "plain_per_matrix_f64": plain_res,
+        "ragged_c128": c128_res,
         "small_batched_c1": {"workload": "pfaffian() 1024 x 32x32 f64", "ms": t_c1 * 1e3,
                              "pf_per_s": 1024 / t_c1,
                              "hbm_frac": (1024 * 32 * 31 / 2 * 8) / t_c1 / HBM_PEAK},
