@@ -51,6 +51,19 @@
 #ifndef PFH_BAT_NB_256
 #define PFH_BAT_NB_256 12
 #endif
+// complex128: twice the LDS per pair.  Measured (ms per bucket, nb 4 / 5-6
+// / 8 / 12): 64-128 0.96 / 0.83 (6) / 1.11 / 1.20; 128-192 3.88 / 2.74 (6) /
+// 2.80; 192-256 7.75 / 7.04 (5) / 5.89 (8); nb = 8 is the widest that keeps
+// two workgroups of NP = 256 on a CU.
+#ifndef PFH_BAT_NB_C128_128
+#define PFH_BAT_NB_C128_128 6
+#endif
+#ifndef PFH_BAT_NB_C128_192
+#define PFH_BAT_NB_C128_192 6
+#endif
+#ifndef PFH_BAT_NB_C128_256
+#define PFH_BAT_NB_C128_256 8
+#endif
 
 namespace pfh {
 
@@ -60,11 +73,30 @@ typedef double bat_d4 __attribute__((ext_vector_type(4)));
 
 // Entry (r, c) of a matrix whose upper triangle is stored with row stride
 // ld; the load is unconditional (the diagonal when r == c).
-__device__ __forceinline__ double bat_w(const double* W, int ld, int r, int c) {
+template <typename T>
+__device__ __forceinline__ T bat_w(const T* W, int ld, int r, int c) {
     const int a = r < c ? r : c, b = r < c ? c : r;
-    const double v = W[(int64_t)a * ld + b];
-    return r < c ? v : (r > c ? -v : 0.0);
+    const T v = W[(int64_t)a * ld + b];
+    return r < c ? v : (r > c ? -v : zero<T>());
 }
+
+template <typename T>
+__device__ __forceinline__ T bat_sel(bool on, T v) { return on ? v : zero<T>(); }
+
+// The same entry split in two, so that a run of loads can be issued before
+// any of them is used: the element index, and a sign code from the indices
+// alone (+1 above the diagonal, -1 below, 0 on it).
+__device__ __forceinline__ int64_t bat_ix(int ld, int r, int c, int& sg) {
+    sg = r < c ? 1 : (r > c ? -1 : 0);
+    const int a = r < c ? r : c, b = r < c ? c : r;
+    return (int64_t)a * ld + b;
+}
+template <typename T>
+__device__ __forceinline__ T bat_apply(int sg, T v) { return sg > 0 ? v : (sg < 0 ? -v : zero<T>()); }
+
+// pivot-search magnitude: |x| for real, |z|^2 for complex (NaN stays NaN)
+__device__ __forceinline__ double bat_mag(double x) { return fabs(x); }
+__device__ __forceinline__ double bat_mag(c128 z) { return z.re * z.re + z.im * z.im; }
 
 // workgroup argmax, part 1: the wave's best (|value|, key) into LDS
 __device__ __forceinline__ void bat_wave_best(double best, int key, double* r_best, int* r_key) {
@@ -75,11 +107,11 @@ __device__ __forceinline__ void bat_wave_best(double best, int key, double* r_be
 
 }  // namespace
 
-template <int NP, int NB>
+template <typename T, int NP, int NB>
 __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
     constexpr int NW = NP / 64;
     extern __shared__ double bat_lds[];     // U_j, V_j of local slot s at [2 (j NP + s)] and [2 (j NP + s) + 1]
-    double* Pp = bat_lds;
+    T* Pp = reinterpret_cast<T*>(bat_lds);
     __shared__ int Qpan[2][2 * NB + 2];     // slot of logical k0 + q, by pair-step parity
     __shared__ int Qs[NP];                  // trailing update: local slot of logical lo + r
     __shared__ double r_best[2][NW];        // per-wave argmax, by pair-step parity
@@ -87,8 +119,9 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
 
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int lr = lane & 15, lq = lane >> 4;
-    double* slab0 = d.bslab + (int64_t)blockIdx.x * 2 * NP * NP;   // this workgroup's slab pair
-    double* slab1 = slab0 + (int64_t)NP * NP;
+    T* slab0 = reinterpret_cast<T*>(d.bslab) + (int64_t)blockIdx.x * 2 * NP * NP;   // this workgroup's slab pair
+    T* slab1 = slab0 + (int64_t)NP * NP;
+    T* phase_out = reinterpret_cast<T*>(d.phase_out);
 
     int64_t count = d.count;
     if (d.count_dev) {
@@ -100,18 +133,18 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
         const int n = __builtin_amdgcn_readfirstlane(d.sizes ? d.sizes[idx] : d.dense_n);
         const int64_t off = d.offsets ? d.offsets[idx] : (int64_t)idx * n * n;
         if (n < 0 || n > NP) {
-            if (t == 0) { d.log_out[idx] = __builtin_nan(""); ((double*)d.phase_out)[idx] = __builtin_nan(""); }
+            if (t == 0) { d.log_out[idx] = __builtin_nan(""); phase_out[idx] = nan_of<T>(); }
             continue;
         }
         if (n == 0 || (n & 1)) {
-            if (t == 0) { d.log_out[idx] = -INFINITY; ((double*)d.phase_out)[idx] = 0.0; }
+            if (t == 0) { d.log_out[idx] = -INFINITY; phase_out[idx] = zero<T>(); }
             continue;
         }
-        const double* src = (const double*)d.data + off;
+        const T* src = reinterpret_cast<const T*>(d.data) + off;
         int ld = n;
-        double* dst = slab0;
+        T* dst = slab0;
         LogAcc la;
-        Phase<double> ph;
+        Phase<T> ph;
         bool dead = false;
 
         for (int k0 = 0; k0 < n; k0 += 2 * NB) {
@@ -124,14 +157,14 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
             __syncthreads();                 // the previous panel's LDS reads are done
             for (int q = t; q < 2 * NB + 2; q += NP) Qpan[0][q] = k0 + q;
 #pragma unroll 4
-            for (int j = 0; j < NB; ++j) { Pp[2 * (j * NP + s)] = 0.0; Pp[2 * (j * NP + s) + 1] = 0.0; }
+            for (int j = 0; j < NB; ++j) { Pp[2 * (j * NP + s)] = zero<T>(); Pp[2 * (j * NP + s) + 1] = zero<T>(); }
             // first pivot row: row k0 of src (nothing pending)
             {
                 double best = -1.0;
-                const double r0v = bat_w(src, ld, k0, xs);
-                const double r = (own && x > k0) ? r0v : 0.0;
+                const T r0v = bat_w(src, ld, k0, xs);
+                const T r = bat_sel(own && x > k0, r0v);
                 Pp[2 * s + 1] = r;                                   // V_0
-                const double m = fabs(r);
+                const double m = bat_mag(r);
                 if (own && x > k0 && m == m) best = m;
                 bat_wave_best(best, lg * 512 + s, r_best[0], r_key[0]);
                 __syncthreads();
@@ -150,10 +183,10 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
                 int p, sp;
                 if (gb >= 0.0) { p = gk >> 9; sp = k0 + (gk & 511); }
                 else { p = k + 1; sp = sa; }                         // no finite candidate (NaN row)
-                const double piv = Pp[2 * (i * NP + (sp - k0)) + 1]; // row k at column p
-                if (piv == 0.0) { dead = true; break; }              // the same decision in every thread
+                const T piv = Pp[2 * (i * NP + (sp - k0)) + 1];      // row k at column p
+                if (is_zero(piv)) { dead = true; break; }            // the same decision in every thread
                 const bool sw = p != k + 1;
-                la.mul(fabs(piv));
+                la.mul(absd(piv));
                 ph.mul(piv);
                 if (sw) ph.flip();
                 // the next step's map: interchange k+1 <-> p
@@ -168,31 +201,31 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
                 const int y1 = sp, y2 = (p == k + 2) ? sa : sb;
                 if (k + 2 >= n) break;                               // last pair of the matrix
 
-                const double w1 = bat_w(src, ld, y1, xs);
-                const double w2 = bat_w(src, ld, y2, xs);
-                const double w12 = bat_w(src, ld, y1, y2);
+                const T w1 = bat_w(src, ld, y1, xs);
+                const T w2 = bat_w(src, ld, y2, xs);
+                const T w12 = bat_w(src, ld, y1, y2);
                 const int s1 = y1 - k0, s2 = y2 - k0;
-                double r1 = 0.0, rl = 0.0, rk = 0.0;
+                T r1 = zero<T>(), rl = zero<T>(), rk = zero<T>();
                 for (int j = 0; j < i; ++j) {
-                    const double uj = Pp[2 * (j * NP + s)], vj = Pp[2 * (j * NP + s) + 1];
-                    const double b1u = Pp[2 * (j * NP + s1)], b1v = Pp[2 * (j * NP + s1) + 1];
-                    const double b2u = Pp[2 * (j * NP + s2)], b2v = Pp[2 * (j * NP + s2) + 1];
+                    const T uj = Pp[2 * (j * NP + s)], vj = Pp[2 * (j * NP + s) + 1];
+                    const T b1u = Pp[2 * (j * NP + s1)], b1v = Pp[2 * (j * NP + s1) + 1];
+                    const T b2u = Pp[2 * (j * NP + s2)], b2v = Pp[2 * (j * NP + s2) + 1];
                     r1 = r1 + b1u * vj - b1v * uj;
                     rl = rl + b2u * vj - b2v * uj;
                     rk = rk + b1u * b2v - b1v * b2u;
                 }
-                const double inv = 1.0 / piv;
-                const double u = (own && lg >= k + 2) ? (w1 + r1) * inv : 0.0;
-                const double vi = Pp[2 * (i * NP + s) + 1];          // V_i at the own slot
+                const T inv = recip(piv);
+                const T u = bat_sel(own && lg >= k + 2, (w1 + r1) * inv);
+                const T vi = Pp[2 * (i * NP + s) + 1];               // V_i at the own slot
                 Pp[2 * (i * NP + s)] = u;                            // U_i
                 if (!has_next) break;
-                const double u_kl = (w12 + rk) * inv;                // U_i at row k+2
-                const double v_kl = Pp[2 * (i * NP + s2) + 1];       // V_i at row k+2
+                const T u_kl = (w12 + rk) * inv;                     // U_i at row k+2
+                const T v_kl = Pp[2 * (i * NP + s2) + 1];            // V_i at row k+2
                 double best = -1.0;
-                double r = 0.0;
+                T r = zero<T>();
                 if (own && lg >= k + 3) {
                     r = w2 + rl + u_kl * vi - v_kl * u;
-                    const double m = fabs(r);
+                    const double m = bat_mag(r);
                     if (m == m) best = m;
                 }
                 Pp[2 * ((i + 1) * NP + s) + 1] = r;                  // V_{i+1} = row k+2
@@ -217,49 +250,71 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
                 while (rem >= mt - I) { rem -= mt - I; ++I; }
                 J = I + rem;
             };
-            auto gather = [&](int I, int J, double (&w4)[4]) {
+            // raw loads now, signs applied where the values are used
+            auto gather = [&](int I, int J, T (&w4)[4], int (&sg4)[4]) {
                 const int scc = Qs[min(16 * J + lr, m - 1)] + k0;
+                int64_t ix[4];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) w4[e] = bat_w(src, ld, Qs[min(16 * I + lq + 4 * e, m - 1)] + k0, scc);
+                for (int e = 0; e < 4; ++e) ix[e] = bat_ix(ld, Qs[min(16 * I + lq + 4 * e, m - 1)] + k0, scc, sg4[e]);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) w4[e] = src[ix[e]];
             };
             int I = 0, J = 0;
-            double wv4[4] = {0.0, 0.0, 0.0, 0.0};
+            T wv4[4] = {zero<T>(), zero<T>(), zero<T>(), zero<T>()};
+            int sg[4] = {0, 0, 0, 0};
             if (wv < ntile) {
                 tile_ij(wv, I, J);
-                gather(I, J, wv4);
+                gather(I, J, wv4, sg);
             }
             for (int tt = wv; tt < ntile; tt += NW) {
                 int In = I, Jn = J;
-                double wn[4] = {0.0, 0.0, 0.0, 0.0};
+                T wn[4] = {zero<T>(), zero<T>(), zero<T>(), zero<T>()};
+                int sgn[4] = {0, 0, 0, 0};
                 if (tt + NW < ntile) {
                     tile_ij(tt + NW, In, Jn);
-                    gather(In, Jn, wn);
+                    gather(In, Jn, wn, sgn);
                 }
                 const int cc = 16 * J + lr;
                 const int sA = Qs[min(16 * I + lr, m - 1)], sB = Qs[min(cc, m - 1)];
                 bat_d4 acc = {0.0, 0.0, 0.0, 0.0};
+                bat_d4 aci = {0.0, 0.0, 0.0, 0.0};     // complex: the imaginary part
                 // unrolled to the panel width so that every operand read is
-                // issued ahead of the products (pairs past np are masked)
+                // issued ahead of the products (pairs past np are masked);
+                // complex products as four real ones:
+                //   Re += Re a Re b - Im a Im b,  Im += Re a Im b + Im a Re b
 #pragma unroll
                 for (int g = 0; g < (2 * NB + 3) / 4; ++g) {
                     const int q = 4 * g + lq;                        // K index: pair q / 2
                     const int j = q >> 1;
                     const bool on = j < np;
                     const int jj = on ? j : 0;
-                    const double a = Pp[2 * (jj * NP + sA) + (q & 1)];          // U_j, V_j at the row
-                    const double bb = Pp[2 * (jj * NP + sB) + ((q & 1) ^ 1)];   // V_j, U_j at the column
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(on ? a : 0.0, on ? ((q & 1) ? -bb : bb) : 0.0, acc,
-                                                               0, 0, 0);
+                    const T a = bat_sel(on, Pp[2 * (jj * NP + sA) + (q & 1)]);         // U_j, V_j at the row
+                    const T b0 = Pp[2 * (jj * NP + sB) + ((q & 1) ^ 1)];              // V_j, U_j at the column
+                    const T b = bat_sel(on, (q & 1) ? -b0 : b0);
+                    if constexpr (is_cplx<T>::value) {
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.re, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, b.im, acc, 0, 0, 0);
+                        aci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.im, aci, 0, 0, 0);
+                        aci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b.re, aci, 0, 0, 0);
+                    } else {
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+                    }
                 }
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int rr = 16 * I + lq + 4 * e;
-                    if (rr < m && cc < m && cc > rr) dst[(int64_t)(lo + rr) * NP + (lo + cc)] = wv4[e] + acc[e];
+                    if (rr < m && cc < m && cc > rr) {
+                        const T w = bat_apply(sg[e], wv4[e]);
+                        T v;
+                        if constexpr (is_cplx<T>::value) v = T{w.re + acc[e], w.im + aci[e]};
+                        else v = w + acc[e];
+                        dst[(int64_t)(lo + rr) * NP + (lo + cc)] = v;
+                    }
                 }
                 I = In;
                 J = Jn;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) wv4[e] = wn[e];
+                for (int e = 0; e < 4; ++e) { wv4[e] = wn[e]; sg[e] = sgn[e]; }
             }
             src = dst;
             ld = NP;
@@ -267,7 +322,10 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
         }
         if (t == 0) {
             d.log_out[idx] = dead ? -INFINITY : la.value();
-            ((double*)d.phase_out)[idx] = dead ? 0.0 : ph.s;
+            T pv;
+            if constexpr (is_cplx<T>::value) pv = T{ph.re, ph.im};
+            else pv = ph.s;
+            phase_out[idx] = dead ? zero<T>() : pv;
         }
     }
 }
@@ -284,39 +342,44 @@ int bat_num_cu() {
     return v;
 }
 
-template <int NP, int NB>
-constexpr size_t bat_lds_bytes() { return sizeof(double) * 2 * NB * NP; }
+template <typename T, int NP, int NB>
+constexpr size_t bat_lds_bytes() { return sizeof(T) * 2 * NB * NP; }
 
 // resident grid (workgroups per CU x CUs); also sets the kernel's LDS limit,
 // which the occupancy query needs
-template <int NP, int NB>
+template <typename T, int NP, int NB>
 int bat_grid() {
     static int g = 0;
     if (g == 0) {
-        (void)hipFuncSetAttribute((const void*)pf_bat_kernel<NP, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)bat_lds_bytes<NP, NB>());
+        (void)hipFuncSetAttribute((const void*)pf_bat_kernel<T, NP, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)bat_lds_bytes<T, NP, NB>());
         int occ = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, pf_bat_kernel<NP, NB>, NP, bat_lds_bytes<NP, NB>()) !=
-                hipSuccess || occ <= 0)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, pf_bat_kernel<T, NP, NB>, NP,
+                                                         bat_lds_bytes<T, NP, NB>()) != hipSuccess ||
+            occ <= 0)
             occ = 1;
         g = occ * bat_num_cu();
     }
     return g;
 }
 
-template <int NP, int NB>
+// doubles of one workgroup's slab pair
+template <typename T, int NP>
+constexpr int64_t bat_per() { return 2 * (int64_t)NP * NP * (int64_t)(sizeof(T) / sizeof(double)); }
+
+template <typename T, int NP, int NB>
 int bat_launch(const BatchDesc& d, hipStream_t s) {
     if (d.count <= 0) return 0;
-    auto kern = pf_bat_kernel<NP, NB>;
-    constexpr size_t bytes = bat_lds_bytes<NP, NB>();
+    auto kern = pf_bat_kernel<T, NP, NB>;
+    constexpr size_t bytes = bat_lds_bytes<T, NP, NB>();
     static bool attr = false;
     if (!attr) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
         if (e != hipSuccess) return (int)e;
         attr = true;
     }
-    int64_t grid = bat_grid<NP, NB>();
-    const int64_t per = 2 * (int64_t)NP * NP;   // one slab pair per workgroup
+    int64_t grid = bat_grid<T, NP, NB>();
+    constexpr int64_t per = bat_per<T, NP>();   // one slab pair per workgroup
     if (!d.bslab || d.bslab_elems < per) return (int)hipErrorInvalidValue;
     if (grid > d.bslab_elems / per) grid = d.bslab_elems / per;
     if (grid > d.count) grid = d.count;
@@ -324,32 +387,43 @@ int bat_launch(const BatchDesc& d, hipStream_t s) {
     return (int)hipGetLastError();
 }
 
-}  // namespace
-
-
-namespace {
-template <int NP, int NB>
+template <typename T, int NP, int NB>
 int64_t bat_need(int64_t count) {
-    const int64_t g = bat_grid<NP, NB>();
-    return (count < g ? count : g) * 2 * NP * NP;
+    const int64_t g = bat_grid<T, NP, NB>();
+    return (count < g ? count : g) * bat_per<T, NP>();
 }
+
 }  // namespace
 
 // doubles of workspace for batches of `count` matrices with n <= maxn: one
-// slab pair (2 NP^2 doubles) per resident workgroup of the widest
+// slab pair (2 NP^2 elements) per resident workgroup of the widest
 // instantiation a bucket can need (the buckets run one after another)
-int64_t bat_ws_elems(int maxn, int64_t count) {
+int64_t bat_ws_elems(int dtype, int maxn, int64_t count) {
     if (maxn <= 64 || count <= 0) return 0;
-    int64_t v = bat_need<128, PFH_BAT_NB_128>(count);
-    if (maxn > 128) { const int64_t b = bat_need<192, PFH_BAT_NB_192>(count); v = b > v ? b : v; }
-    if (maxn > 192) { const int64_t c = bat_need<256, PFH_BAT_NB_256>(count); v = c > v ? c : v; }
+    if (dtype == DT_C128) {
+        int64_t v = bat_need<c128, 128, PFH_BAT_NB_C128_128>(count);
+        if (maxn > 128) { const int64_t b = bat_need<c128, 192, PFH_BAT_NB_C128_192>(count); v = b > v ? b : v; }
+        if (maxn > 192) { const int64_t c = bat_need<c128, 256, PFH_BAT_NB_C128_256>(count); v = c > v ? c : v; }
+        return v;
+    }
+    if (dtype != DT_F64) return 0;
+    int64_t v = bat_need<double, 128, PFH_BAT_NB_128>(count);
+    if (maxn > 128) { const int64_t b = bat_need<double, 192, PFH_BAT_NB_192>(count); v = b > v ? b : v; }
+    if (maxn > 192) { const int64_t c = bat_need<double, 256, PFH_BAT_NB_256>(count); v = c > v ? c : v; }
     return v;
 }
 
-int launch_bat_f64(const BatchDesc& d, int maxn, hipStream_t s) {
-    if (maxn <= 128) return bat_launch<128, PFH_BAT_NB_128>(d, s);
-    if (maxn <= 192) return bat_launch<192, PFH_BAT_NB_192>(d, s);
-    if (maxn <= 256) return bat_launch<256, PFH_BAT_NB_256>(d, s);
+int launch_bat(int dtype, const BatchDesc& d, int maxn, hipStream_t s) {
+    if (dtype == DT_C128) {
+        if (maxn <= 128) return bat_launch<c128, 128, PFH_BAT_NB_C128_128>(d, s);
+        if (maxn <= 192) return bat_launch<c128, 192, PFH_BAT_NB_C128_192>(d, s);
+        if (maxn <= 256) return bat_launch<c128, 256, PFH_BAT_NB_C128_256>(d, s);
+        return (int)hipErrorInvalidValue;
+    }
+    if (dtype != DT_F64) return (int)hipErrorInvalidValue;
+    if (maxn <= 128) return bat_launch<double, 128, PFH_BAT_NB_128>(d, s);
+    if (maxn <= 192) return bat_launch<double, 192, PFH_BAT_NB_192>(d, s);
+    if (maxn <= 256) return bat_launch<double, 256, PFH_BAT_NB_256>(d, s);
     return (int)hipErrorInvalidValue;
 }
 

@@ -25,7 +25,7 @@ struct BatchDesc {
     void* grad_out;           // MODE_GRAD: 1/2 A^-T, laid out like the input
     double* slab;             // f64 slog, 192 < n <= 256: one spill slab per resident workgroup
     int64_t slab_elems;       //   (lds_spill_elems() doubles each), nullptr when absent
-    double* bslab;            // f64 slog, batched blocked kernel: slab pairs of the resident
+    double* bslab;            // f64 / c128 slog, batched blocked kernel: slab pairs of the resident
     int64_t bslab_elems;      //   workgroups (doubles available), nullptr when absent
 };
 
@@ -45,12 +45,13 @@ int lds_max_n();                 // 192 without spill slabs
 int64_t lds_spill_elems();       // doubles per spill slab
 int lds_spill_slabs();           // slabs needed (one per resident workgroup)
 
-// FP64 batched blocked kernel (pf_bat.hip): MODE_SLOG, n <= 256, one
-// workgroup per matrix over a pair of global NP x NP slabs per resident
+// Batched blocked kernel (pf_bat.hip), f64 and c128: MODE_SLOG, n <= 256,
+// one workgroup per matrix over a pair of global NP x NP slabs per resident
 // workgroup, taken from d.bslab (d.bslab_elems doubles; the grid shrinks to
-// what fits, bat_ws_elems(maxn, count) holds a full one for such a batch).
-int launch_bat_f64(const BatchDesc& d, int maxn, hipStream_t stream);
-int64_t bat_ws_elems(int maxn, int64_t count);
+// what fits, bat_ws_elems(dtype, maxn, count) holds a full one for such a
+// batch).
+int launch_bat(int dtype, const BatchDesc& d, int maxn, hipStream_t stream);
+int64_t bat_ws_elems(int dtype, int maxn, int64_t count);
 
 // Launches the workgroup kernel whose capacity covers `maxn` for every
 // matrix in the batch.  Returns 0 or a hipError_t value.

@@ -144,7 +144,7 @@ int bucket_edges(int dt, int32_t* e) {
         case PFH_F64: e[0] = 32; e[1] = 64; e[2] = 128; e[3] = 192; e[4] = 256; return 5;
         case PFH_F32: e[0] = 32; e[1] = 64; e[2] = 128; e[3] = 256; return 4;
         case PFH_C64: e[0] = 32; e[1] = 64; e[2] = 128; e[3] = 192; return 4;
-        case PFH_C128: e[0] = 32; e[1] = 64; e[2] = 128; return 3;
+        case PFH_C128: e[0] = 32; e[1] = 64; e[2] = 128; e[3] = 192; e[4] = 256; return 5;
     }
     return 0;
 }
@@ -175,8 +175,8 @@ RaggedLayout ragged_layout(int dt, int64_t count, int32_t max_n) {
     if (dt == PFH_F64 && max_n > lds_max_n())   // spill slabs of the LDS kernel's 192 < n <= 256 bucket
         off = align_up(off + sizeof(double) * (size_t)lds_spill_elems() * lds_spill_slabs());
     L.bat = off;
-    if (dt == PFH_F64 && max_n > 64)            // slab pairs of the batched blocked kernel
-        off = align_up(off + sizeof(double) * (size_t)bat_ws_elems(max_n, count));
+    if ((dt == PFH_F64 || dt == PFH_C128) && max_n > 64)   // slab pairs of the batched blocked kernel
+        off = align_up(off + sizeof(double) * (size_t)bat_ws_elems(dt, max_n, count));
     L.slabs = off;
     L.glb_grid = 0;
     L.slab_elems = 0;
@@ -233,9 +233,9 @@ int ragged_typed(int dt, const void* data, const int64_t* offsets, const int32_t
             d.slab = (double*)(ws + L.spill);
             d.slab_elems = lds_spill_elems();
         }
-        if (dt == PFH_F64 && max_n > 64) {
+        if ((dt == PFH_F64 || dt == PFH_C128) && max_n > 64) {
             d.bslab = (double*)(ws + L.bat);
-            d.bslab_elems = bat_ws_elems(max_n, count);
+            d.bslab_elems = bat_ws_elems(dt, max_n, count);
         }
         if (b >= nb) {
             err = launch_glb<T>(d, (T*)(ws + L.slabs), L.slab_elems, L.glb_grid, s);
@@ -267,8 +267,8 @@ int pfaffian_dense(int dt, const void* A, int64_t batch, int32_t n, void* out, h
 // blocked kernel's slab pairs (n > 64), then the LDS kernel's spill slabs
 // (n > 192).
 size_t dense_bat_bytes(int dt, int64_t batch, int32_t n) {
-    if (dt != PFH_F64 || n <= 64 || n > reg_max_n(dt)) return 0;
-    return align_up(sizeof(double) * (size_t)bat_ws_elems(n, batch));
+    if ((dt != PFH_F64 && dt != PFH_C128) || n <= 64 || n > reg_max_n(dt)) return 0;
+    return align_up(sizeof(double) * (size_t)bat_ws_elems(dt, n, batch));
 }
 size_t dense_spill_bytes(int dt, int32_t n) {
     if (dt != PFH_F64 || n <= lds_max_n() || n > reg_max_n(dt)) return 0;
@@ -303,7 +303,7 @@ int slog_dense(int dt, const T* A, int64_t batch, int32_t n, int method, double*
             const size_t bb = dense_bat_bytes(dt, batch, n);
             if (bb) {
                 d.bslab = (double*)ws;
-                d.bslab_elems = bat_ws_elems(n, batch);
+                d.bslab_elems = bat_ws_elems(dt, n, batch);
             }
             if (dense_spill_bytes(dt, n)) {
                 d.slab = (double*)((char*)ws + bb);

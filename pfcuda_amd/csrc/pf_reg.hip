@@ -272,7 +272,7 @@ int reg_max_n(int dtype) {
         case DT_F64: return 256;
         case DT_F32: return 256;
         case DT_C64: return 192;
-        case DT_C128: return 128;
+        case DT_C128: return 256;   // slog above 128 only through the batched blocked kernel (d.bslab)
     }
     return 0;
 }
@@ -396,7 +396,18 @@ int launch_reg(int dtype, int mode, const BatchDesc& d, int maxn, hipStream_t s)
     if (dtype == DT_F64) {
         const int kf = f64_kernel(mode, maxn, d);
         if (kf == 2) return launch_lds_f64(mode, d, maxn, s);
-        if (kf == 3) return launch_bat_f64(d, maxn, s);
+        if (kf == 3) return launch_bat(DT_F64, d, maxn, s);
+    }
+    if (dtype == DT_C128 && mode == MODE_SLOG && maxn > 64) {
+        // complex128 slog: the batched blocked kernel above 64 (the lanes
+        // kernel holds at most 128; PFH_C128_KERNEL=lanes keeps it there)
+        static int lanes = -1;
+        if (lanes < 0) {
+            const char* e = getenv("PFH_C128_KERNEL");
+            lanes = (e && strcmp(e, "lanes") == 0) ? 1 : 0;
+        }
+        if (d.bslab && !(lanes && maxn <= 128)) return launch_bat(DT_C128, d, maxn, s);
+        if (maxn > 128) return (int)hipErrorInvalidValue;
     }
     if (mode == MODE_GRAD) {
         switch (dtype) {
