@@ -83,10 +83,11 @@ class Ragged:
         self.stream = torch.cuda.current_stream(dev).cuda_stream
 
     def __call__(self):
+        # the current stream at call time, so that a graph capture records it
         st = self.lib.pfh_slog_pfaffian_ragged(
             self.dt, self.data.data_ptr(), self.off.data_ptr(), self.siz.data_ptr(), self.count,
             self.max_n, self.log.data_ptr(), self.ph.data_ptr(), self.ws.data_ptr(), self.ws.numel(),
-            self.stream)
+            self.torch.cuda.current_stream().cuda_stream)
         if st != 0:
             raise RuntimeError(f"pfh_slog_pfaffian_ragged status {st}")
 
@@ -166,6 +167,24 @@ def main():
         print(f"# profile mode: {args.steps} ragged steps and one 1073741824-byte copy done", file=sys.stderr,
               flush=True)
         return
+
+    # the same call captured in a graph and replayed (checked against eager)
+    want_log = run.log.clone()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        run()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        run()
+    graph.replay()
+    torch.cuda.synchronize()
+    graph_ok = bool(torch.equal(run.log, want_log))
+    ms_graph = time_events(torch, graph.replay, args.steps) * 1e3
+    print(f"# ragged graph replay: {ms_graph:.3f} ms/step, equal to eager: {graph_ok}", file=sys.stderr, flush=True)
+    del graph
 
     flops = float(np.sum(sizes.astype(np.float64) ** 3) / 3.0)
     tri_bytes = float(np.sum(sizes.astype(np.float64) * (sizes - 1) / 2.0) * 8)
@@ -399,6 +418,8 @@ def main():
                      "launch_ms": t_big * 1e3, "algorithmic_flop": big_flops},
         "whole_batch": {"flop": flops, "tflops": flops / (ms / 1e3) / 1e12,
                         "frac_fp64_peak": flops / (ms / 1e3) / FP64_PEAK, "triangle_bytes": tri_bytes},
+        "graph_replay": {"ms_per_step": ms_graph, "pf_per_s": args.count / (ms_graph / 1e3),
+                         "equal_to_eager": graph_ok},
         "plain_per_matrix_f64": plain_res,
         "ragged_c128": c128_res,
         "small_batched_c1": {"workload": "pfaffian() 1024 x 32x32 f64", "ms": t_c1 * 1e3,

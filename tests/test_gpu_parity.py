@@ -201,6 +201,55 @@ def test_blocked_dies_mid_panel(pf, cuda, dtype):
     assert l0 == -math.inf and p0 == 0
 
 
+def test_ragged_graph_replay(pf, cuda):
+    # The ragged call (device-side bucketing plus one launch per bucket, no
+    # memsets, no host round trip) captured in a graph: every replay gives
+    # the eager result bit for bit, with eager work between the replays.
+    import ctypes
+    from pfcuda_amd import _lib
+    rng = np.random.default_rng(12)
+    for dt, tdt in ((_lib.PFH_F64, torch.float64), (_lib.PFH_C128, torch.complex128)):
+        sizes = (2 * rng.integers(8, 129, size=160)).astype(np.int32)
+        mats = [oracle.skew_gaussian(rng, int(n), complex_=tdt.is_complex) for n in sizes]
+        data = torch.from_numpy(np.concatenate([m.reshape(-1) for m in mats])).to(cuda)
+        offs = torch.from_numpy(np.cumsum([0] + [int(n) ** 2 for n in sizes[:-1]]).astype(np.int64)).to(cuda)
+        siz = torch.from_numpy(sizes).to(cuda)
+        count, max_n = len(sizes), int(sizes.max())
+        need = ctypes.c_size_t(0)
+        _lib.check(_lib.lib.pfh_slog_pfaffian_ragged_workspace_size(dt, count, max_n, need), "workspace")
+        ws = torch.empty(max(int(need.value), 1), dtype=torch.uint8, device=cuda)
+        lo = torch.empty(count, dtype=torch.float64, device=cuda)
+        ph = torch.empty(count, dtype=tdt, device=cuda)
+
+        def call():
+            _lib.check(_lib.lib.pfh_slog_pfaffian_ragged(
+                dt, data.data_ptr(), offs.data_ptr(), siz.data_ptr(), count, max_n, lo.data_ptr(),
+                ph.data_ptr(), ws.data_ptr(), ws.numel(), torch.cuda.current_stream().cuda_stream), "ragged")
+
+        call()
+        torch.cuda.synchronize()
+        want_l, want_p = lo.clone(), ph.clone()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            call()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            call()
+        for rep in range(3):
+            lo.fill_(0)
+            ph.fill_(0)
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(lo, want_l) and torch.equal(ph, want_p), (tdt, rep)
+            call()
+            torch.cuda.synchronize()
+        wl, wp = oracle.slog_ragged(data.cpu().numpy(), offs.cpu().numpy(), sizes, threads=8)
+        assert np.max(np.abs(want_l.cpu().numpy() - wl)) < 1e-9
+
+
 def test_batched_dies_at_first_and_mid_panel(pf, cuda):
     # f64 slog at 64 < n <= 256 (the batched blocked kernel): an all-zero
     # matrix dies at the first pair step; 50 zero rows/columns scattered in an
