@@ -1,5 +1,5 @@
-// Batched blocked Parlett-Reid, FP64 slog: one workgroup per matrix, with
-// several workgroups resident per CU.
+// Batched blocked Parlett-Reid, slog in FP64 and complex128: one workgroup
+// per matrix, with several workgroups resident per CU.
 //
 // The LDS-resident kernel (pf_lds.hip) keeps the whole triangle in LDS, so
 // above n = 128 one matrix occupies a CU and its serial pair-step chain is
@@ -21,8 +21,8 @@
 //   trailing update (once per panel): C = stale[Q r][Q c] + A B with
 //     A[r][2j] = U_j, A[r][2j+1] = V_j, B[2j][c] = V_j, B[2j+1][c] = -U_j
 //     (the two terms of a pair adjacent in K), one v_mfma_f64_16x16x4f64
-//     per 16 x 16 x 4 step, operands from LDS, written compacted into the
-//     other slab.
+//     per 16 x 16 x 4 step (four per complex step), operands from LDS,
+//     written compacted into the other slab.
 //
 // The arithmetic is the skew Schur complement S = A22 + (y x^T - x y^T)/a of
 // the reference's pf_sm_calc (src/pfaffian_sm.cu:71-89) and the lazy row
