@@ -992,6 +992,445 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
     return (int)hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Gradient of log|Pf| for one large matrix: a blocked block sweep
+// (Gauss-Jordan with the 2x2 Parlett-Reid pivots) that leaves A^-1 in pivot
+// order, from which grad = 1/2 A^-T = -1/2 A^-1 (A^-1 is skew).
+//
+// Per group of np pair steps, with G = [k0, k0 + 2np) the group's positions
+// after its interchanges and T every other position:
+//   bp_panel     chooses the group's pivots on the unswept block [k0, n),
+//                which is exactly the factorization's Schur complement, so
+//                the factorization's panel serves unchanged (perm: logical
+//                -> slot on [k0, n); positions below k0 keep their place)
+//   gj_inv       Ai = A_GG^-1 by the unblocked sweep of the 2np x 2np block
+//                (its pivots are already in place: no search)
+//   gj_rows      S = A_GT and Z = Ai A_GT at every column of T, in logical
+//                order; at the columns of G, S = -I and Z = -Ai
+//   gj_trailing  Wn[r][c] = base + sum_h S[h][r] Z[h][c] over every upper
+//                tile, base = W[perm r][perm c] for r, c in T, else 0
+// This is the principal pivot transform of the matrix on G,
+//   new_GG = A_GG^-1,  new_GT = -A_GG^-1 A_GT,  new_TT = A_TT - A_TG A_GG^-1 A_GT,
+// i.e. the group's np sweeps of the unblocked form at once: one read and
+// one write of the upper triangle per group instead of per pair step.
+// `gp` composes the groups' interchanges (logical -> caller's index).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool gj_in(int x, int k0, int m) { return x >= k0 && x < k0 + m; }
+
+// position x of the current matrix in the previous one
+__device__ __forceinline__ int gj_src(const int* perm, int x, int k0) {
+    const int p = perm[x];
+    return x < k0 ? x : p;
+}
+
+// 1024 threads; thread t holds the entries (t / TPR, t % TPR + TPR i) of the
+// small block in registers.  Per pair step the two pivot rows go through LDS
+// (double-buffered by step parity: one barrier per step).
+template <typename T, int NB>
+__global__ void __launch_bounds__(1024) gj_inv(const T* __restrict__ W, const int* __restrict__ perm, int64_t n,
+                                               int k0, int np, const BPState<T>* st, T* __restrict__ Ai) {
+    constexpr int M = 2 * NB, TPR = 1024 / M, EPT = M / TPR;
+    __shared__ T xr[2][M], yr[2][M];
+    __shared__ int pg[M];
+    if (st->dead) return;
+    const int m = 2 * np, t = threadIdx.x;
+    const int r = t / TPR, cq = t % TPR;
+    if (t < M) pg[t] = perm[k0 + min(t, m - 1)];
+    __syncthreads();
+    T v[EPT];
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) {
+        const int c = cq + TPR * i;
+        const T w = w_at(W, n, pg[min(r, m - 1)], pg[min(c, m - 1)]);
+        v[i] = (r < m && c < m) ? w : zero<T>();
+    }
+    for (int j = 0; j < np; ++j) {
+        const int k = 2 * j, par = j & 1;
+        if (r == k || r == k + 1) {
+#pragma unroll
+            for (int i = 0; i < EPT; ++i) {
+                if (r == k) xr[par][cq + TPR * i] = v[i];
+                else yr[par][cq + TPR * i] = v[i];
+            }
+        }
+        __syncthreads();
+        const T inv = recip(xr[par][k + 1]);
+        const T xrr = xr[par][r], yrr = yr[par][r];
+        const bool pr = r == k || r == k + 1;
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) {
+            const int c = cq + TPR * i;
+            const T xc = xr[par][c], yc = yr[par][c];
+            const bool pc = c == k || c == k + 1;
+            T u;
+            if (pr && pc) u = (r == c) ? zero<T>() : (r == k ? -inv : inv);
+            else if (pr) u = (r == k) ? yc * inv : -(xc * inv);
+            else if (pc) u = (c == k) ? -(yrr * inv) : xrr * inv;
+            else u = v[i] + (yrr * xc - xrr * yc) * inv;
+            if (r < m && c < m) v[i] = u;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < EPT; ++i) Ai[r * M + cq + TPR * i] = v[i];
+}
+
+template <typename T, int NB>
+__global__ void __launch_bounds__(256) gj_rows(const T* __restrict__ W, const int* __restrict__ perm, int64_t n,
+                                               int k0, int np, const BPState<T>* st, const T* __restrict__ Ai,
+                                               T* __restrict__ S, T* __restrict__ Z, const int* __restrict__ gp_old,
+                                               int* __restrict__ gp_new) {
+    constexpr int M = 2 * NB, CB = 32, HPT = M / 8;
+    __shared__ T As[M][M + 1];
+    __shared__ T Sc[M][CB + 1];
+    if (st->dead) return;
+    const int m = 2 * np, t = threadIdx.x;
+    const int x0 = blockIdx.x * CB;
+    if (t < CB && x0 + t < n) gp_new[x0 + t] = gp_old[gj_src(perm, x0 + t, k0)];
+    for (int e = t; e < M * M; e += 256) As[e / M][e % M] = Ai[e];
+    for (int e = t; e < M * CB; e += 256) {
+        const int h = e / CB, cx = e % CB, x = x0 + cx;
+        T v = zero<T>();
+        if (h < m && x < n) {
+            if (gj_in(x, k0, m)) v = (x - k0 == h) ? -one<T>() : zero<T>();
+            else v = w_at(W, n, perm[k0 + h], gj_src(perm, x, k0));
+        }
+        Sc[h][cx] = v;
+    }
+    __syncthreads();
+    const int cx = t % CB, hg = t / CB, x = x0 + cx;
+    T acc[HPT];
+#pragma unroll
+    for (int i = 0; i < HPT; ++i) acc[i] = zero<T>();
+    if (gj_in(x, k0, m)) {
+#pragma unroll
+        for (int i = 0; i < HPT; ++i) acc[i] = -As[hg + 8 * i][x - k0];
+    } else {
+        for (int hp = 0; hp < m; ++hp) {
+            const T sv = Sc[hp][cx];
+#pragma unroll
+            for (int i = 0; i < HPT; ++i) acc[i] = acc[i] + As[hg + 8 * i][hp] * sv;
+        }
+    }
+    if (x < n) {
+#pragma unroll
+        for (int i = 0; i < HPT; ++i) {
+            const int h = hg + 8 * i;
+            if (h < m) { S[(int64_t)h * n + x] = Sc[h][cx]; Z[(int64_t)h * n + x] = acc[i]; }
+        }
+    }
+}
+
+// upper 64 x 64 tile index -> (ti, tj), tj >= ti
+__device__ __forceinline__ void gj_tile(int ntile, int& ti, int& tj) {
+    int tb = blockIdx.x;
+    ti = 0;
+    while (tb >= ntile - ti) { tb -= ntile - ti; ++ti; }
+    tj = ti + tb;
+}
+
+// any element type: thread (tx, ty) owns rows r0+ty+16a, cols c0+tx+16b
+template <typename T>
+__global__ void __launch_bounds__(256) gj_trailing(const T* __restrict__ Wo, T* __restrict__ Wn,
+                                                   const T* __restrict__ S, const T* __restrict__ Z,
+                                                   const int* __restrict__ perm, int64_t n, int k0, int m, int ntile,
+                                                   const BPState<T>* st) {
+    if (st->dead) return;
+    int ti, tj;
+    gj_tile(ntile, ti, tj);
+    const int r0 = ti * BP_TILE, c0 = tj * BP_TILE;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4, n1 = (int)n - 1;
+    int rr[4], cc[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) { rr[x] = min(r0 + ty + 16 * x, n1); cc[x] = min(c0 + tx + 16 * x, n1); }
+    T acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = zero<T>();
+    for (int h = 0; h < m; ++h) {
+        T sr[4], zc[4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x) { sr[x] = S[(int64_t)h * n + rr[x]]; zc[x] = Z[(int64_t)h * n + cc[x]]; }
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) acc[x][y] = acc[x][y] + sr[x] * zc[y];
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+        const int r = r0 + ty + 16 * x;
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const int c = c0 + tx + 16 * y;
+            if (c < n && c > r) {
+                T b = zero<T>();
+                if (!gj_in(r, k0, m) && !gj_in(c, k0, m)) b = w_at(Wo, n, gj_src(perm, r, k0), gj_src(perm, c, k0));
+                Wn[(int64_t)r * n + c] = b + acc[x][y];
+            }
+        }
+    }
+}
+
+// FP64 form on the matrix cores: C = S^T Z per 64 x 64 upper tile with
+// v_mfma_f64_16x16x4f64 (lane layout as in bp_trailing_mfma: A operand
+// A[l % 16][l / 16] = S[k][r], B operand B[l / 16][l % 16] = Z[k][c],
+// accumulator register e holds (l / 16 + 4 e, l % 16)).  S and Z are in
+// logical order, so their reads are coalesced and need no map.
+template <int M>
+__global__ void __launch_bounds__(256) gj_trailing_mfma(const double* __restrict__ Wo, double* __restrict__ Wn,
+                                                        const double* __restrict__ S, const double* __restrict__ Z,
+                                                        const int* __restrict__ perm, int64_t n, int k0, int m,
+                                                        int ntile, const BPState<double>* st) {
+    if (st->dead) return;
+    int ti, tj;
+    gj_tile(ntile, ti, tj);
+    const int r0 = ti * BP_TILE + 16 * (threadIdx.x >> 6), c0 = tj * BP_TILE;
+    const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+    const int n1 = (int)n - 1;
+    int pr[4], pc[4];
+    bool tr[4], tc[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int r = min(r0 + lq + 4 * e, n1);
+        pr[e] = gj_src(perm, r, k0);
+        tr[e] = !gj_in(r, k0, m);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int c = min(c0 + 16 * u + lr, n1);
+        pc[u] = gj_src(perm, c, k0);
+        tc[u] = !gj_in(c, k0, m);
+    }
+    const int ra = min(r0 + lr, n1);
+    double av[M / 4], bv[M / 4][4];
+#pragma unroll
+    for (int g = 0; g < M / 4; ++g) {
+        const int q = 4 * g + lq;
+        const bool on = q < m;
+        const double va = S[(int64_t)q * n + ra];
+        av[g] = on ? va : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double vb = Z[(int64_t)q * n + min(c0 + 16 * u + lr, n1)];
+            bv[g][u] = on ? vb : 0.0;
+        }
+    }
+    double wv[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const double w = w_at(Wo, n, pr[e], pc[u]);
+            wv[u][e] = (tr[e] && tc[u]) ? w : 0.0;
+        }
+    bt_d4 acc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] = bt_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int g = 0; g < M / 4; ++g) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[g], bv[g][u], acc[u], 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = r0 + lq + 4 * e, c = c0 + 16 * u + lr;
+            if (c < n && c > r) Wn[(int64_t)r * n + c] = wv[u][e] + acc[u][e];
+        }
+    }
+}
+
+// complex128 form: each complex product as four real ones
+template <int M>
+__global__ void __launch_bounds__(256) gj_trailing_mfma_c(const c128* __restrict__ Wo, c128* __restrict__ Wn,
+                                                          const c128* __restrict__ S, const c128* __restrict__ Z,
+                                                          const int* __restrict__ perm, int64_t n, int k0, int m,
+                                                          int ntile, const BPState<c128>* st) {
+    if (st->dead) return;
+    int ti, tj;
+    gj_tile(ntile, ti, tj);
+    const int r0 = ti * BP_TILE + 16 * (threadIdx.x >> 6), c0 = tj * BP_TILE;
+    const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+    const int n1 = (int)n - 1;
+    int pr[4], pc[4];
+    bool tr[4], tc[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int r = min(r0 + lq + 4 * e, n1);
+        pr[e] = gj_src(perm, r, k0);
+        tr[e] = !gj_in(r, k0, m);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int c = min(c0 + 16 * u + lr, n1);
+        pc[u] = gj_src(perm, c, k0);
+        tc[u] = !gj_in(c, k0, m);
+    }
+    const int ra = min(r0 + lr, n1);
+    bt_d4 are[4], aim[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { are[u] = bt_d4{0.0, 0.0, 0.0, 0.0}; aim[u] = bt_d4{0.0, 0.0, 0.0, 0.0}; }
+#pragma unroll
+    for (int g = 0; g < M / 4; ++g) {
+        const int q = 4 * g + lq;
+        const bool on = q < m;
+        const c128 va = S[(int64_t)q * n + ra];
+        const double ar = on ? va.re : 0.0, ai = on ? va.im : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const c128 vb = Z[(int64_t)q * n + min(c0 + 16 * u + lr, n1)];
+            const double br = on ? vb.re : 0.0, bi = on ? vb.im : 0.0;
+            are[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, are[u], 0, 0, 0);
+            are[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, are[u], 0, 0, 0);
+            aim[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, aim[u], 0, 0, 0);
+            aim[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, aim[u], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = r0 + lq + 4 * e, c = c0 + 16 * u + lr;
+            const c128 w = w_at(Wo, n, pr[e], pc[u]);
+            const bool b = tr[e] && tc[u];
+            if (c < n && c > r) Wn[(int64_t)r * n + c] = c128{(b ? w.re : 0.0) + are[u][e], (b ? w.im : 0.0) + aim[u][e]};
+        }
+    }
+}
+
+// (skipped once the matrix is dead or a barrier failed: the maps may then
+// be unset, and gj_out reads neither)
+template <typename T>
+__global__ void gj_invperm(const int* __restrict__ gp, int* __restrict__ ig, int64_t n, const BPState<T>* st) {
+    if (st->dead || st->err) return;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x)
+        ig[gp[x]] = (int)x;
+}
+
+// G[i][j] = -1/2 A^-1[i][j]: the swept matrix read through the inverse of
+// the composed interchanges (the writes are coalesced)
+template <typename T>
+__global__ void __launch_bounds__(256) gj_out(const T* __restrict__ W, const int* __restrict__ ig, int64_t n,
+                                              const BPState<T>* st, T* __restrict__ G) {
+    using R = typename real_of<T>::type;
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= n * n) return;
+    if (st->dead || st->err) { G[e] = nan_of<T>(); return; }
+    const int64_t i = e / n, j = e % n;
+    G[e] = w_at(W, n, ig[i], ig[j]) * (R)(-0.5);
+}
+
+template <typename T>
+struct GJLayout {
+    BPLayout<T> bp;
+    size_t s, z, ai, gp0, gp1, ig, total;
+};
+
+template <typename T>
+static GJLayout<T> gj_layout(int64_t n, int nb) {
+    GJLayout<T> L{};
+    L.bp = bp_layout<T>(n, nb);
+    size_t o = L.bp.total;
+    L.s = o; o = al(o + sizeof(T) * (size_t)2 * nb * n);
+    L.z = o; o = al(o + sizeof(T) * (size_t)2 * nb * n);
+    L.ai = o; o = al(o + sizeof(T) * (size_t)4 * nb * nb);
+    L.gp0 = o; o = al(o + sizeof(int) * (size_t)n);
+    L.gp1 = o; o = al(o + sizeof(int) * (size_t)n);
+    L.ig = o; o = al(o + sizeof(int) * (size_t)n);
+    L.total = o;
+    return L;
+}
+
+// The blocked sweep needs the cooperative panel over the whole first
+// block; PFH_GRAD_BLOCKED=0 selects the unblocked sweep (read per call).
+bool blocked_grad_enabled(int64_t n) {
+    const char* e = std::getenv("PFH_GRAD_BLOCKED");
+    if (e && e[0] == '0') return false;
+    return coop_enabled() && (n + BC_T - 1) / BC_T <= BC_MAXG && n % 2 == 0;
+}
+
+size_t blocked_grad_workspace_bytes(int dtype, int64_t n) {
+    const int nb = blocked_panel_width(dtype);
+    switch (dtype) {
+        case DT_F32: return gj_layout<float>(n, nb).total;
+        case DT_F64: return gj_layout<double>(n, nb).total;
+        case DT_C64: return gj_layout<c64>(n, nb).total;
+        case DT_C128: return gj_layout<c128>(n, nb).total;
+    }
+    return 0;
+}
+
+template <typename T>
+int blocked_grad(const T* A, int64_t n, void* ws, double* log_out, T* phase_out, T* G, hipStream_t s) {
+    constexpr int NB = std::is_same<T, c128>::value ? 16 : 32;
+    if (n % 2 != 0 || (n + BC_T - 1) / BC_T > BC_MAXG) return (int)hipErrorInvalidValue;
+    const GJLayout<T> L = gj_layout<T>(n, NB);
+    char* base = (char*)ws;
+    T* W[2] = {(T*)(base + L.bp.w0), (T*)(base + L.bp.w1)};
+    BPArgs<T> a{};
+    a.P = (T*)(base + L.bp.p);
+    a.R = (T*)(base + L.bp.r);
+    a.perm = (int*)(base + L.bp.perm);
+    a.part_v = (double*)(base + L.bp.pv);
+    a.part_i = (int*)(base + L.bp.pi);
+    a.st = (BPState<T>*)(base + L.bp.st);
+    a.n = n;
+    a.nb = NB;
+    BPCoop<T> co{};
+    co.part = (BPPart<T>*)(base + L.bp.part);
+    co.cand = (T*)(base + L.bp.cand);
+    co.flags = (int*)(base + L.bp.bar);
+    T* S = (T*)(base + L.s);
+    T* Z = (T*)(base + L.z);
+    T* Ai = (T*)(base + L.ai);
+    int* gp[2] = {(int*)(base + L.gp0), (int*)(base + L.gp1)};
+    int* ig = (int*)(base + L.ig);
+    hipError_t e = hipMemcpyAsync(W[0], A, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return (int)e;
+    a.W = W[0];
+    hipLaunchKernelGGL(bp_init<T>, dim3(64), dim3(256), 0, s, a, co.flags);
+    hipLaunchKernelGGL(bp_reset_perm<T>, dim3(64), dim3(256), 0, s, gp[0], (int64_t)0, n);
+    const int ntile = (int)((n + BP_TILE - 1) / BP_TILE);
+    const int tiles = ntile * (ntile + 1) / 2;
+    int cur = 0, gc = 0, epoch = 0;
+    for (int64_t k0 = 0; k0 + 1 < n; k0 += 2 * NB) {
+        const int np = (int)((n - k0) / 2 < NB ? (n - k0) / 2 : NB);
+        const int m = 2 * np;
+        a.W = W[cur];
+        e = launch_panel<T, NB>(a, co, (int)k0, np, epoch, (int)((n - k0 + BC_T - 1) / BC_T), s);
+        if (e != hipSuccess) return (int)e;
+        epoch += np;
+        hipLaunchKernelGGL((gj_inv<T, NB>), dim3(1), dim3(1024), 0, s, W[cur], a.perm, n, (int)k0, np, a.st, Ai);
+        hipLaunchKernelGGL((gj_rows<T, NB>), dim3((unsigned)((n + 31) / 32)), dim3(256), 0, s, W[cur], a.perm, n,
+                           (int)k0, np, a.st, Ai, S, Z, gp[gc], gp[gc ^ 1]);
+        if constexpr (std::is_same<T, double>::value) {
+            hipLaunchKernelGGL(gj_trailing_mfma<2 * NB>, dim3(tiles), dim3(256), 0, s, W[cur], W[cur ^ 1], S, Z,
+                               a.perm, n, (int)k0, m, ntile, a.st);
+        } else if constexpr (std::is_same<T, c128>::value) {
+            hipLaunchKernelGGL(gj_trailing_mfma_c<2 * NB>, dim3(tiles), dim3(256), 0, s, W[cur], W[cur ^ 1], S, Z,
+                               a.perm, n, (int)k0, m, ntile, a.st);
+        } else {
+            hipLaunchKernelGGL(gj_trailing<T>, dim3(tiles), dim3(256), 0, s, W[cur], W[cur ^ 1], S, Z, a.perm, n,
+                               (int)k0, m, ntile, a.st);
+        }
+        cur ^= 1;
+        gc ^= 1;
+    }
+    hipLaunchKernelGGL(bp_finish<T>, dim3(1), dim3(64), 0, s, a.st, log_out, phase_out);
+    hipLaunchKernelGGL(gj_invperm<T>, dim3(64), dim3(256), 0, s, gp[gc], ig, n, a.st);
+    hipLaunchKernelGGL(gj_out<T>, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, s, W[cur], ig, n, a.st, G);
+    return (int)hipGetLastError();
+}
+
+#define PFH_INST(T) \
+    template int blocked_grad<T>(const T*, int64_t, void*, double*, T*, T*, hipStream_t);
+PFH_INST(float)
+PFH_INST(double)
+PFH_INST(c64)
+PFH_INST(c128)
+#undef PFH_INST
+
 #define PFH_INST(T) \
     template int blocked_slog<T>(const T*, int64_t, int, bool, void*, double*, T*, hipStream_t);
 PFH_INST(float)

@@ -338,8 +338,11 @@ int slog_dense(int dt, const T* A, int64_t batch, int32_t n, int method, double*
     return 0;
 }
 
+// Matrices past the register kernels: the blocked sweep when the device
+// takes its cooperative panel, else the unblocked one.
 size_t grad_ws_bytes(int dt, int32_t n) {
     if (n <= reg_grad_max_n(dt)) return 0;
+    if (blocked_grad_enabled(n)) return blocked_grad_workspace_bytes(dt, n);
     return align_up(dtype_bytes(dt) * (size_t)n * n) + align_up(sizeof(int) * (size_t)n) +
            align_up(sizeof(LargeState<c128>));
 }
@@ -360,6 +363,14 @@ int grad_dense(int dt, const T* A, int64_t batch, int32_t n, double* log_out, T*
     void* ws = nullptr;
     int err = resolve_ws(ws_in, ws_bytes, grad_ws_bytes(dt, n), &ws);
     if (err) return err;
+    if (blocked_grad_enabled(n)) {
+        for (int64_t i = 0; i < batch; ++i) {
+            const int64_t off = i * (int64_t)n * n;
+            err = blocked_grad<T>(A + off, n, ws, log_out + i, phase_out + i, G + off, s);
+            if (err) return err;
+        }
+        return 0;
+    }
     char* b = (char*)ws;
     T* W = (T*)b;
     int* perm = (int*)(b + align_up(sizeof(T) * (size_t)n * n));

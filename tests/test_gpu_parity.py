@@ -311,3 +311,68 @@ def test_grad_autograd_and_finite_differences(pf, cuda):
         fm = oracle.slog(A0 - h * E)[0]
         fd = (fp - fm) / (2 * h)
         assert abs((G[i, j] - G[j, i]) - fd) < 1e-6 * max(1.0, abs(fd)), (i, j, G[i, j] - G[j, i], fd)
+
+
+@pytest.mark.parametrize("blocked", ["1", "0"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
+@pytest.mark.parametrize("n", [520, 1000])
+def test_grad_large_sweeps_vs_inverse(pf, cuda, monkeypatch, dtype, n, blocked):
+    # The blocked block sweep (groups of 32 pairs, 16 for complex128; at
+    # n = 1000 the last group is partial) and the unblocked one
+    # (PFH_GRAD_BLOCKED=0) against numpy's inverse and the oracle's slog.
+    from pfcuda_amd.adjoint import slog_grad
+    monkeypatch.setenv("PFH_GRAD_BLOCKED", blocked)
+    rng = np.random.default_rng(300 + n)
+    A = oracle.skew_gaussian(rng, n, complex_=dtype.is_complex)
+    G, la, ph = slog_grad(torch.from_numpy(A).to(cuda), return_slog=True)
+    want = 0.5 * np.linalg.inv(A).T
+    err = np.abs(G.cpu().numpy() - want).max() / np.abs(want).max()
+    assert err < 1e-10, (n, blocked, err)
+    l0, p0 = oracle.slog(A)
+    assert abs(float(la) - l0) < 1e-9 and abs(complex(ph.item()) - p0) < 1e-9
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.complex64])
+def test_grad_large_single_precision(pf, cuda, dtype):
+    # the blocked sweep's plain trailing update (no matrix cores)
+    from pfcuda_amd.adjoint import slog_grad
+    rng = np.random.default_rng(31)
+    n = 300
+    A = oracle.skew_gaussian(rng, n, complex_=dtype.is_complex)
+    G = slog_grad(torch.from_numpy(A).to(cuda).to(dtype)).cpu().numpy()
+    want = 0.5 * np.linalg.inv(A).T
+    assert np.abs(G - want).max() / np.abs(want).max() < 1e-2
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
+@pytest.mark.parametrize("zero_from", [0, 500])
+def test_grad_large_singular(pf, cuda, dtype, zero_from):
+    # an exactly singular matrix: the sweep stops at its zero pivot (in the
+    # first group, or after 250 pair steps) and returns (-inf, 0) and a NaN
+    # gradient, as the unblocked sweep does
+    from pfcuda_amd.adjoint import slog_grad
+    rng = np.random.default_rng(78)
+    n = 600
+    A = oracle.skew_gaussian(rng, n, complex_=dtype.is_complex)
+    A[zero_from:, :] = 0
+    A[:, zero_from:] = 0
+    perm = rng.permutation(n)
+    A = np.ascontiguousarray(A[perm][:, perm])
+    G, la, ph = slog_grad(torch.from_numpy(A).to(cuda), return_slog=True)
+    assert float(la) == -math.inf and complex(ph.item()) == 0
+    assert torch.isnan(G).all()
+
+
+def test_grad_n2048_residual(pf, cuda):
+    # C5 at full size: A G^T = I/2 (G = 1/2 A^-T), and the slog against the oracle
+    from pfcuda_amd.adjoint import slog_grad
+    rng = np.random.default_rng(2048)
+    n = 2048
+    A = oracle.skew_gaussian(rng, n)
+    At = torch.from_numpy(A).to(cuda)
+    G, la, ph = slog_grad(At, return_slog=True)
+    R = At @ G.T - 0.5 * torch.eye(n, dtype=torch.float64, device=cuda)
+    rel = float(R.abs().max()) / (float(At.abs().max()) * float(G.abs().max()) * n)
+    assert rel < 1e-13, rel
+    l0, p0 = oracle.slog(A)
+    assert abs(float(la) - l0) < 1e-8 and float(ph) == p0
