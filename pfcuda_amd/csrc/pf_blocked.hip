@@ -325,6 +325,12 @@ __global__ void __launch_bounds__(BP_THREADS) bp_row(BPArgs<T> a, int k, int i, 
 // double-buffered by pair-step parity.
 // ---------------------------------------------------------------------------
 
+// U_j, V_j of one slot, adjacent in the panel's LDS store
+template <typename T>
+struct alignas(2 * sizeof(T)) BCPair {
+    T u, v;
+};
+
 template <typename T>
 struct BPPart {
     double best;
@@ -554,12 +560,25 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         const T inv = recip(piv);
         // corrections of the pairs j < i, in chunks of eight (entries past i
         // are zero in b1, b2 and Ps, so a partial chunk adds nothing)
+        // (real types: the own slot's eight pairs are read first, each as
+        // one aligned pair, so the LDS reads go out back to back from one
+        // base address; complex pairs are read as they are used, which keeps
+        // the kernel within its registers)
         T r1 = zero<T>(), rl = zero<T>(), rk = zero<T>();
+        const BCPair<T>* Pq = reinterpret_cast<const BCPair<T>*>(Ps);
         for (int j0 = 0; j0 < i; j0 += 8) {
+            constexpr int PRE = is_cplx<T>::value ? 1 : 8;
+            BCPair<T> pv[PRE];
+            if constexpr (PRE == 8) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) pv[q] = Pq[(j0 + q) * BC_T + t];
+            }
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const int j = j0 + q;
-                const T uj = Ps[2 * (j * BC_T + t)], vj = Ps[2 * (j * BC_T + t) + 1];
+                T uj, vj;
+                if constexpr (PRE == 8) { uj = pv[q].u; vj = pv[q].v; }
+                else { uj = Ps[2 * (j * BC_T + t)]; vj = Ps[2 * (j * BC_T + t) + 1]; }
                 const T b1u = b1[2 * j], b1v = b1[2 * j + 1];
                 const T b2u = b2[2 * j], b2v = b2[2 * j + 1];
                 r1 = r1 + b1u * vj - b1v * uj;
