@@ -825,6 +825,110 @@ __global__ void __launch_bounds__(256) bp_trailing_mfma_c(const c128* __restrict
     }
 }
 
+// ---------------------------------------------------------------------------
+// Trailing updates with the B operand staged in LDS.  The four waves of a
+// 64 x 64 tile share its K x 64 B operand: here it is read once per tile
+// from L2 into LDS (16 values per thread) instead of once per wave, and each
+// wave keeps only its own A rows in registers.  The stale-entry gathers are
+// issued after the staging stores and before the barrier, so they overlap
+// the barrier and the products.  Same products,
+// same K order and same lane layout as the register forms above.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void tr_mma(double a, double b, bt_d4& re, bt_d4& im) {
+    (void)im;
+    re = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, re, 0, 0, 0);
+}
+__device__ __forceinline__ void tr_mma(c128 a, c128 b, bt_d4& re, bt_d4& im) {
+    re = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.re, re, 0, 0, 0);
+    re = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, b.im, re, 0, 0, 0);
+    im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.im, im, 0, 0, 0);
+    im = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b.re, im, 0, 0, 0);
+}
+__device__ __forceinline__ double tr_out(double w, const bt_d4& re, const bt_d4& im, int e) {
+    (void)im;
+    return w + re[e];
+}
+__device__ __forceinline__ c128 tr_out(c128 w, const bt_d4& re, const bt_d4& im, int e) {
+    return c128{w.re + re[e], w.im + im[e]};
+}
+
+// Factorization: Wn[r][c] = Wo[q r][q c] + sum_j (U_j V_j' - V_j U_j') over
+// the upper tiles of [lo, n)^2 (as bp_trailing_mfma / bp_trailing_mfma_c).
+template <typename T, int NB>
+__global__ void __launch_bounds__(256) bp_trailing_staged(const T* __restrict__ Wo, T* __restrict__ Wn,
+                                                          const T* __restrict__ P, const int* __restrict__ perm,
+                                                          int64_t n, int lo, int np, int ntile, int pslot) {
+    constexpr int K = 2 * NB;
+    __shared__ T Bs[K][BP_TILE];
+    int tb = blockIdx.x, ti = 0;
+    while (tb >= ntile - ti) { tb -= ntile - ti; ++ti; }
+    const int tj = ti + tb;
+    const int r0 = lo + ti * BP_TILE + 16 * (threadIdx.x >> 6), c0 = lo + tj * BP_TILE;
+    const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+    const int col = threadIdx.x & 63, kq = threadIdx.x >> 6;   // staging: column, first K row
+    const int n1 = (int)n - 1;
+    int pr[4], pc[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) pr[e] = perm[min(r0 + lq + 4 * e, n1)];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pc[u] = perm[min(c0 + 16 * u + lr, n1)];
+    const int pa = perm[min(r0 + lr, n1)];
+    const int pcs = perm[min(c0 + col, n1)];
+    const int sa = pslot ? pa : min(r0 + lr, n1);
+    const int sbs = pslot ? pcs : min(c0 + col, n1);
+    T sv[K / 4];   // B[kq + 4 i][col]: V_j, then -U_j
+#pragma unroll
+    for (int i = 0; i < K / 4; ++i) {
+        const int q = kq + 4 * i, j = q >> 1;
+        const T v = P[(int64_t)((q & 1) ? j : NB + j) * n + sbs];
+        sv[i] = (j < np) ? ((q & 1) ? -v : v) : zero<T>();
+    }
+    T av[K / 4];   // A[r][4 g + lq]: U_j, V_j
+#pragma unroll
+    for (int g = 0; g < K / 4; ++g) {
+        const int q = 4 * g + lq, j = q >> 1;
+        const T v = P[(int64_t)((q & 1) ? NB + j : j) * n + sa];
+        av[g] = (j < np) ? v : zero<T>();
+    }
+#pragma unroll
+    for (int i = 0; i < K / 4; ++i) Bs[kq + 4 * i][col] = sv[i];
+    T wv[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) wv[u][e] = w_at(Wo, n, pr[e], pc[u]);
+    __syncthreads();
+    bt_d4 re[4], im[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { re[u] = bt_d4{0.0, 0.0, 0.0, 0.0}; im[u] = bt_d4{0.0, 0.0, 0.0, 0.0}; }
+#pragma unroll
+    for (int g = 0; g < K / 4; ++g) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) tr_mma(av[g], Bs[4 * g + lq][16 * u + lr], re[u], im[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = r0 + lq + 4 * e, c = c0 + 16 * u + lr;
+            if (c < n && c > r) Wn[(int64_t)r * n + c] = tr_out(wv[u][e], re[u], im[u], e);
+        }
+    }
+}
+
+// Which form runs, by measurement (ms, staged / register): the f64
+// factorization keeps the register form (N = 4096: 14.2 / 13.9, N = 8192:
+// 35.6 / 34.8); its c128 form (N = 4096: 21.7 / 22.3) and the block sweep
+// (f64 N = 2048: 8.0 / 8.6, N = 4096: 18.5 / 22.5; c128 N = 4096: 30.5 /
+// 32.1) use the staged form.  PFH_TRAIL_STAGED=1 or 0 forces one form
+// everywhere (read per call).
+static bool trail_staged(bool dflt) {
+    const char* e = std::getenv("PFH_TRAIL_STAGED");
+    if (e && e[0] == '0') return false;
+    if (e && e[0] == '1') return true;
+    return dflt;
+}
+
 template <typename T>
 __global__ void bp_finish(const BPState<T>* st, double* log_out, T* phase_out) {
     if (threadIdx.x != 0) return;
@@ -981,7 +1085,10 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
                 if (e != hipSuccess) return (int)e;
             }
             if constexpr (std::is_same<T, double>::value) {
-                if (nb == 32) {
+                if (nb == 32 && trail_staged(false)) {
+                    hipLaunchKernelGGL((bp_trailing_staged<double, 32>), dim3(tiles), dim3(256), 0, s, W[cur],
+                                       W[cur ^ 1], a.P, a.perm, n, (int)lo, np, ntile, panel_coop ? 1 : 0);
+                } else if (nb == 32) {
                     hipLaunchKernelGGL(bp_trailing_mfma<32>, dim3(tiles), dim3(256), 0, s, W[cur], W[cur ^ 1], a.P,
                                        a.perm, n, (int)lo, np, ntile, panel_coop ? 1 : 0);
                 } else {
@@ -989,7 +1096,10 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
                                        a.perm, n, (int)lo, nb, np, ntile, panel_coop ? 1 : 0);
                 }
             } else if constexpr (std::is_same<T, c128>::value) {
-                if (nb == 16) {
+                if (nb == 16 && trail_staged(true)) {
+                    hipLaunchKernelGGL((bp_trailing_staged<c128, 16>), dim3(tiles), dim3(256), 0, s, W[cur],
+                                       W[cur ^ 1], a.P, a.perm, n, (int)lo, np, ntile, panel_coop ? 1 : 0);
+                } else if (nb == 16) {
                     hipLaunchKernelGGL(bp_trailing_mfma_c<16>, dim3(tiles), dim3(256), 0, s, W[cur], W[cur ^ 1],
                                        a.P, a.perm, n, (int)lo, np, ntile, panel_coop ? 1 : 0);
                 } else {
@@ -1318,6 +1428,79 @@ __global__ void __launch_bounds__(256) gj_trailing_mfma_c(const c128* __restrict
     }
 }
 
+// Block sweep: Wn[r][c] = base + sum_h S[h][r] Z[h][c] over every upper
+// tile (as gj_trailing_mfma / gj_trailing_mfma_c).
+template <typename T, int M>
+__global__ void __launch_bounds__(256) gj_trailing_staged(const T* __restrict__ Wo, T* __restrict__ Wn,
+                                                          const T* __restrict__ S, const T* __restrict__ Z,
+                                                          const int* __restrict__ perm, int64_t n, int k0, int m,
+                                                          int ntile, const BPState<T>* st) {
+    __shared__ T Bs[M][BP_TILE];
+    if (st->dead) return;
+    int ti, tj;
+    gj_tile(ntile, ti, tj);
+    const int r0 = ti * BP_TILE + 16 * (threadIdx.x >> 6), c0 = tj * BP_TILE;
+    const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+    const int col = threadIdx.x & 63, kq = threadIdx.x >> 6;
+    const int n1 = (int)n - 1;
+    int pr[4], pc[4];
+    bool tr[4], tc[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int r = min(r0 + lq + 4 * e, n1);
+        pr[e] = gj_src(perm, r, k0);
+        tr[e] = !gj_in(r, k0, m);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int c = min(c0 + 16 * u + lr, n1);
+        pc[u] = gj_src(perm, c, k0);
+        tc[u] = !gj_in(c, k0, m);
+    }
+    const int ra = min(r0 + lr, n1), cs = min(c0 + col, n1);
+    T sv[M / 4];
+#pragma unroll
+    for (int i = 0; i < M / 4; ++i) {
+        const int q = kq + 4 * i;
+        const T v = Z[(int64_t)q * n + cs];
+        sv[i] = q < m ? v : zero<T>();
+    }
+    T av[M / 4];
+#pragma unroll
+    for (int g = 0; g < M / 4; ++g) {
+        const int q = 4 * g + lq;
+        const T v = S[(int64_t)q * n + ra];
+        av[g] = q < m ? v : zero<T>();
+    }
+#pragma unroll
+    for (int i = 0; i < M / 4; ++i) Bs[kq + 4 * i][col] = sv[i];
+    T wv[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const T w = w_at(Wo, n, pr[e], pc[u]);
+            wv[u][e] = (tr[e] && tc[u]) ? w : zero<T>();
+        }
+    __syncthreads();
+    bt_d4 re[4], im[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { re[u] = bt_d4{0.0, 0.0, 0.0, 0.0}; im[u] = bt_d4{0.0, 0.0, 0.0, 0.0}; }
+#pragma unroll
+    for (int g = 0; g < M / 4; ++g) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) tr_mma(av[g], Bs[4 * g + lq][16 * u + lr], re[u], im[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = r0 + lq + 4 * e, c = c0 + 16 * u + lr;
+            if (c < n && c > r) Wn[(int64_t)r * n + c] = tr_out(wv[u][e], re[u], im[u], e);
+        }
+    }
+}
+
 // (skipped once the matrix is dead or a barrier failed: the maps may then
 // be unset, and gj_out reads neither)
 template <typename T>
@@ -1423,6 +1606,15 @@ int blocked_grad(const T* A, int64_t n, void* ws, double* log_out, T* phase_out,
         hipLaunchKernelGGL((gj_inv<T, NB>), dim3(1), dim3(1024), 0, s, W[cur], a.perm, n, (int)k0, np, a.st, Ai);
         hipLaunchKernelGGL((gj_rows<T, NB>), dim3((unsigned)((n + 31) / 32)), dim3(256), 0, s, W[cur], a.perm, n,
                            (int)k0, np, a.st, Ai, S, Z, gp[gc], gp[gc ^ 1]);
+        if constexpr (std::is_same<T, double>::value || std::is_same<T, c128>::value) {
+            if (trail_staged(true)) {
+                hipLaunchKernelGGL((gj_trailing_staged<T, 2 * NB>), dim3(tiles), dim3(256), 0, s, W[cur], W[cur ^ 1],
+                                   S, Z, a.perm, n, (int)k0, m, ntile, a.st);
+                cur ^= 1;
+                gc ^= 1;
+                continue;
+            }
+        }
         if constexpr (std::is_same<T, double>::value) {
             hipLaunchKernelGGL(gj_trailing_mfma<2 * NB>, dim3(tiles), dim3(256), 0, s, W[cur], W[cur ^ 1], S, Z,
                                a.perm, n, (int)k0, m, ntile, a.st);
