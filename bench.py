@@ -7,7 +7,8 @@ and resident in HBM before timing.  One step = one pfh_slog_pfaffian_ragged
 call over the whole batch (device-side size bucketing + one persistent launch
 per bucket).  Prints one JSON line.
 
-Also reported: the small batched product path (1024 x 32x32 f64); the
+Also reported: the small batched product path (1024 x 32x32 f64, eager and
+replayed from a captured graph, and the same kernel over 2^20 matrices); the
 dominant kernel's roofline fraction (HIP events on the launch stream) with
 its HBM bytes per launch from the committed rocprofv3 counter summary;
 single large matrices (N = 4096 f64 and c128, plain vs blocked; the N = 8192
@@ -266,10 +267,29 @@ def main():
     out1 = torch.empty(1024, dtype=torch.float64, device=dev)
 
     def c1():
-        lib.pfh_pfaffian_f64(A1.data_ptr(), 1024, 32, out1.data_ptr(), strm)
+        # the current stream at call time, so that a graph capture records it
+        lib.pfh_pfaffian_f64(A1.data_ptr(), 1024, 32, out1.data_ptr(), torch.cuda.current_stream().cuda_stream)
     c1()
     torch.cuda.synchronize()
     t_c1 = time_events(torch, c1, 20)
+    # the 1024-matrix call captured in a graph and replayed (launch cost
+    # amortized; checked against eager)
+    want1 = out1.clone()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        c1()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g1 = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g1):
+        c1()
+    out1.zero_()
+    g1.replay()
+    torch.cuda.synchronize()
+    c1_graph_ok = bool(torch.equal(out1, want1))
+    t_c1g = time_events(torch, g1.replay, 200)
+    del g1
 
     # single large matrices: plain (unblocked) vs blocked Parlett-Reid
     def large(n, dtype, seed, reps=1):
@@ -376,6 +396,25 @@ def main():
                 "n2048_ms": t_g2 * 1e3, "n2048_rel_err_vs_inv": err_g2}
     print(f"# c5: {grad_res}", file=sys.stderr, flush=True)
 
+    # C1's kernel over 2^20 matrices (8.6 GB resident), last among the GPU
+    # measurements: the large transient allocation is not in the way of the others
+    big = 1 << 20
+    g.manual_seed(args.seed + 11)
+    Bb = torch.randn(big, 32, 32, generator=g, device=dev, dtype=torch.float64)
+    Abig = ((Bb - Bb.transpose(1, 2)) / 2).contiguous()
+    del Bb
+    outb = torch.empty(big, dtype=torch.float64, device=dev)
+
+    def c1big():
+        lib.pfh_pfaffian_f64(Abig.data_ptr(), big, 32, outb.data_ptr(), strm)
+    c1big()
+    torch.cuda.synchronize()
+    c1_big_finite = bool(torch.isfinite(outb).all())
+    t_c1b = time_events(torch, c1big, 5)
+    del Abig, outb
+    torch.cuda.empty_cache()
+    print(f"# c1: 1024 x 32^2 {t_c1 * 1e3:.4f} ms (graph replay {t_c1g * 1e3:.4f} ms, equal {c1_graph_ok}); "
+          f"2^20 x 32^2 {t_c1b * 1e3:.3f} ms", file=sys.stderr, flush=True)
     # accuracy of a sample against the FP64 CPU oracle, and its CPU time
     res = {}
     cpu_baseline = None
@@ -424,7 +463,15 @@ def main():
         "ragged_c128": c128_res,
         "small_batched_c1": {"workload": "pfaffian() 1024 x 32x32 f64", "ms": t_c1 * 1e3,
                              "pf_per_s": 1024 / t_c1,
-                             "hbm_frac": (1024 * 32 * 31 / 2 * 8) / t_c1 / HBM_PEAK},
+                             "hbm_frac": (1024 * 32 * 31 / 2 * 8) / t_c1 / HBM_PEAK,
+                             "graph_replay_ms": t_c1g * 1e3, "graph_replay_pf_per_s": 1024 / t_c1g,
+                             "graph_equal_to_eager": c1_graph_ok,
+                             "batch_2p20_ms": t_c1b * 1e3, "batch_2p20_pf_per_s": big / t_c1b,
+                             "batch_2p20_hbm_frac": (big * 32 * 31 / 2 * 8) / t_c1b / HBM_PEAK,
+                             "batch_2p20_read_frac": (big * 32 * 32 * 8) / t_c1b / HBM_PEAK,
+                             "batch_2p20_all_finite": c1_big_finite,
+                             "note": "hbm_frac counts the strict upper triangle (algorithmic bytes); "
+                                     "read_frac the full square the kernel reads"},
         "large_single_matrix": large_res,
         "gradient_c5": grad_res,
         "accuracy_vs_oracle": res,

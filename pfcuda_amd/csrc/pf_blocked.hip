@@ -975,17 +975,19 @@ int blocked_panel_width(int dtype) {
 }
 
 // The cooperative panel kernel runs when the device supports cooperative
-// launches (PFH_BLOCKED_COOP=0 turns it off) and the panel's columns fit
-// BC_MAXG blocks of BC_T slots.
+// launches and has at least BC_MAXG CUs (so that a panel's blocks, one per
+// CU, fit at once), unless PFH_BLOCKED_COOP=0, and while the panel's
+// columns fit BC_MAXG blocks of BC_T slots.
 static bool coop_enabled() {
     static int v = -1;
     if (v < 0) {
         const char* e = std::getenv("PFH_BLOCKED_COOP");
-        int dev = 0, ok = 0;
+        int dev = 0, ok = 0, cus = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&ok, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess)
+            hipDeviceGetAttribute(&ok, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             ok = 0;
-        v = (ok && !(e && e[0] == '0')) ? 1 : 0;
+        v = (ok && cus >= BC_MAXG && !(e && e[0] == '0')) ? 1 : 0;
     }
     return v == 1;
 }
@@ -1000,6 +1002,21 @@ static hipError_t launch_panel(const BPArgs<T>& a, const BPCoop<T>& co, int k0, 
                                            (int)smem);
         if (e != hipSuccess) return e;
         attr = true;
+    }
+    // Plain launch by default: G <= BC_MAXG blocks at one per CU (their LDS)
+    // on a device with more CUs than that are resident together once the
+    // stream's earlier work has drained.  A block that cannot become
+    // resident (the CUs held by other work for seconds) ends in the
+    // barrier's bounded spin, whose error makes the result NaN; nothing
+    // hangs.  PFH_PANEL_LAUNCH=coop asks the runtime for the co-residency
+    // guarantee instead (hipLaunchCooperativeKernel), at a measured cost of
+    // ~15 us per panel in a fresh process and ~100 us per panel in one that
+    // has captured and replayed graphs on several streams (N = 4096 f64:
+    // 12.6 ms plain, 13.5 / 20.5 ms cooperative).
+    const char* lm = std::getenv("PFH_PANEL_LAUNCH");
+    if (!(lm && lm[0] == 'c')) {
+        hipLaunchKernelGGL((bp_panel<T, NB>), dim3(G), dim3(BC_T), smem, s, a, co, k0, np, epoch0);
+        return hipGetLastError();
     }
     BPArgs<T> a_ = a;
     BPCoop<T> co_ = co;

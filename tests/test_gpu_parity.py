@@ -201,6 +201,38 @@ def test_blocked_dies_mid_panel(pf, cuda, dtype):
     assert l0 == -math.inf and p0 == 0
 
 
+def test_pfaffian_graph_replay(pf, cuda):
+    # The dense product call (one kernel launch) captured in a graph: every
+    # replay equals the eager result bit for bit, with eager work between.
+    from pfcuda_amd import _lib
+    rng = np.random.default_rng(13)
+    A = oracle.skew_gaussian(rng, 32, batch=1024)
+    At = torch.from_numpy(A).to(cuda)
+    out = torch.empty(1024, dtype=torch.float64, device=cuda)
+
+    def call():
+        _lib.check(_lib.lib.pfh_pfaffian_f64(At.data_ptr(), 1024, 32, out.data_ptr(),
+                                             torch.cuda.current_stream().cuda_stream), "pfaffian")
+    call()
+    torch.cuda.synchronize()
+    want = out.clone()
+    np.testing.assert_allclose(want[:16].cpu().numpy(), [oracle.pf_f64(a)[0] for a in A[:16]], rtol=1e-12)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        call()
+    torch.cuda.current_stream().wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        call()
+    for _ in range(3):
+        out.fill_(0.0)
+        (At @ At).sum().item()                 # eager work between replays
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, want)
+
+
 def test_ragged_graph_replay(pf, cuda):
     # The ragged call (device-side bucketing plus one launch per bucket, no
     # memsets, no host round trip) captured in a graph: every replay gives
@@ -383,6 +415,25 @@ def test_trailing_forms_bitwise_equal(pf, cuda, monkeypatch, dtype):
     assert np.array_equal(res["1"][2], res["0"][2])
     l0, p0 = oracle.slog(A)
     assert abs(res["1"][0] - l0) < 1e-9 and abs(res["1"][1] - p0) < 1e-9
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
+def test_panel_launch_forms_bitwise_equal(pf, cuda, monkeypatch, dtype):
+    # The panel kernel launched plainly (the default) and through the
+    # runtime's cooperative launch (PFH_PANEL_LAUNCH=coop) is the same code
+    # on the same grid: results agree bit for bit.
+    from pfcuda_amd.adjoint import slog_grad
+    rng = np.random.default_rng(62)
+    A = oracle.skew_gaussian(rng, 700, complex_=dtype.is_complex)
+    At = torch.from_numpy(A).to(cuda)
+    res = {}
+    for form in ("plain", "coop"):
+        monkeypatch.setenv("PFH_PANEL_LAUNCH", form)
+        la, s = pf.slog_pfaffian(At, method=pf.PFH_METHOD_BLOCKED)
+        G = slog_grad(At)
+        res[form] = (float(la), complex(s.item()), G.cpu().numpy())
+    assert res["plain"][0] == res["coop"][0] and res["plain"][1] == res["coop"][1]
+    assert np.array_equal(res["plain"][2], res["coop"][2])
 
 
 def test_grad_n2048_residual(pf, cuda):
