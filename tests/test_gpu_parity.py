@@ -233,6 +233,58 @@ def test_pfaffian_graph_replay(pf, cuda):
         assert torch.equal(out, want)
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
+def test_blocked_and_grad_graph_replay(pf, cuda, dtype):
+    # The blocked slog and the large gradient enqueue a device-to-device copy
+    # and plain kernel launches only (workspace from the caller), so both can
+    # be captured into one graph: every replay equals the eager result bit
+    # for bit, with eager work between the replays.
+    import ctypes
+    from pfcuda_amd import _lib
+    lib = _lib.lib
+    dt = _lib.PFH_C128 if dtype.is_complex else _lib.PFH_F64
+    rng = np.random.default_rng(63)
+    n = 600
+    A0 = oracle.skew_gaussian(rng, n, complex_=dtype.is_complex)
+    A = torch.from_numpy(A0).to(cuda)
+    need, gneed = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    _lib.check(lib.pfh_slog_pfaffian_workspace_size(dt, 1, n, _lib.PFH_METHOD_BLOCKED, need), "workspace")
+    _lib.check(lib.pfh_slog_pfaffian_grad_workspace_size(dt, 1, n, gneed), "grad workspace")
+    ws = torch.empty(max(int(need.value), 1), dtype=torch.uint8, device=cuda)
+    gws = torch.empty(max(int(gneed.value), 1), dtype=torch.uint8, device=cuda)
+    lo = torch.empty(1, dtype=torch.float64, device=cuda)
+    ph = torch.empty(1, dtype=dtype, device=cuda)
+    glo = torch.empty(1, dtype=torch.float64, device=cuda)
+    gph = torch.empty(1, dtype=dtype, device=cuda)
+    G = torch.empty_like(A)
+
+    def call():
+        s = torch.cuda.current_stream().cuda_stream
+        _lib.check(lib.pfh_slog_pfaffian_method(dt, A.data_ptr(), 1, n, _lib.PFH_METHOD_BLOCKED, lo.data_ptr(),
+                                                ph.data_ptr(), ws.data_ptr(), ws.numel(), s), "slog")
+        _lib.check(lib.pfh_slog_pfaffian_grad(dt, A.data_ptr(), 1, n, glo.data_ptr(), gph.data_ptr(), G.data_ptr(),
+                                              gws.data_ptr(), gws.numel(), s), "grad")
+    call()
+    torch.cuda.synchronize()
+    want = (lo.clone(), ph.clone(), G.clone())
+    l0, p0 = oracle.slog(A0)
+    assert abs(float(want[0]) - l0) < 1e-9 and abs(complex(want[1].item()) - p0) < 1e-9
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        call()
+    torch.cuda.current_stream().wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        call()
+    for _ in range(3):
+        lo.fill_(0.0); ph.fill_(0.0); G.fill_(0.0)
+        (A @ A).abs().sum().item()             # eager work between replays
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(lo, want[0]) and torch.equal(ph, want[1]) and torch.equal(G, want[2])
+
+
 def test_ragged_graph_replay(pf, cuda):
     # The ragged call (device-side bucketing plus one launch per bucket, no
     # memsets, no host round trip) captured in a graph: every replay gives
