@@ -77,6 +77,14 @@ __device__ __forceinline__ T w_at(const T* W, int64_t n, int pr, int pc) {
     return pr < pc ? v : (pr > pc ? -v : zero<T>());
 }
 
+// c ? a : b, with complex values selected part by part (a select of whole
+// structs can go through scratch memory)
+template <typename T>
+__device__ __forceinline__ T psel(bool c, T a, T b) {
+    if constexpr (is_cplx<T>::value) return T{c ? a.re : b.re, c ? a.im : b.im};
+    else return c ? a : b;
+}
+
 template <typename T>
 __device__ __forceinline__ T wp(const T* W, const int* perm, int64_t n, int r, int c) {
     return w_at(W, n, perm[r], perm[c]);
@@ -484,7 +492,7 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
                 const int g = wv + BC_WAVES * mm;
                 const bool ok = g < G && lane_ok;
                 const T v = cb[ok ? (int64_t)g * 2 * NB + lane : 0];
-                cv[mm] = ok ? v : zero<T>();
+                cv[mm] = psel(ok, v, zero<T>());
             }
         }
         const int xs = own ? x : k0;           // a valid slot for masked reads
@@ -497,8 +505,8 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
             const T* src = (isv ? PV : PU) + (int64_t)j * n;
             const T v1 = src[sa], v2 = src[sbs];
             if (t < 2 * NB) {
-                Bx[1][t] = ok ? v1 : zero<T>();
-                Bx[2][t] = (ok && sb >= 0) ? v2 : zero<T>();
+                Bx[1][t] = psel(ok, v1, zero<T>());
+                Bx[2][t] = psel(ok && sb >= 0, v2, zero<T>());
             }
         }
         T w2 = wsl(a.W, n, sbs, xs);
@@ -523,8 +531,12 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         __syncthreads();
         int p, sp;
         T piv;
-        if (wb >= 0) { p = s_p; sp = s_sp; piv = s_piv; }
-        else { p = k + 1; sp = sa; piv = Bx[1][2 * i + 1]; }   // no finite candidate (NaN row)
+        {
+            const T pw = s_piv, pn = Bx[1][2 * i + 1];
+            if (wb >= 0) { p = s_p; sp = s_sp; }
+            else { p = k + 1; sp = sa; }                   // no finite candidate (NaN row)
+            piv = psel(wb >= 0, pw, pn);
+        }
         if (is_zero(piv)) {                                // same decision in every block
             if (keeper) st->dead = 1;
             return;
@@ -1200,20 +1212,25 @@ __global__ void __launch_bounds__(1024) gj_inv(const T* __restrict__ W, const in
             }
         }
         __syncthreads();
+        // branch-free: every LDS read of the step is issued before any use,
+        // and each entry's new value is selected from all the candidates
+        T xc[EPT], yc[EPT];
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) { xc[i] = xr[par][cq + TPR * i]; yc[i] = yr[par][cq + TPR * i]; }
         const T inv = recip(xr[par][k + 1]);
         const T xrr = xr[par][r], yrr = yr[par][r];
         const bool pr = r == k || r == k + 1;
+        const T ninv = -inv;
 #pragma unroll
         for (int i = 0; i < EPT; ++i) {
             const int c = cq + TPR * i;
-            const T xc = xr[par][c], yc = yr[par][c];
             const bool pc = c == k || c == k + 1;
-            T u;
-            if (pr && pc) u = (r == c) ? zero<T>() : (r == k ? -inv : inv);
-            else if (pr) u = (r == k) ? yc * inv : -(xc * inv);
-            else if (pc) u = (c == k) ? -(yrr * inv) : xrr * inv;
-            else u = v[i] + (yrr * xc - xrr * yc) * inv;
-            if (r < m && c < m) v[i] = u;
+            const T gen = v[i] + (yrr * xc[i] - xrr * yc[i]) * inv;
+            const T rowv = psel(r == k, yc[i] * inv, -(xc[i] * inv));
+            const T colv = psel(c == k, -(yrr * inv), xrr * inv);
+            const T pp = psel(r == c, zero<T>(), psel(r == k, ninv, inv));
+            const T u = psel(pr, psel(pc, pp, rowv), psel(pc, colv, gen));
+            v[i] = psel(r < m && c < m, u, v[i]);
         }
     }
 #pragma unroll
