@@ -47,6 +47,7 @@
 
 #include "pf_batch.h"
 #include "pf_regstore.h"
+#include "pf_wave.h"
 
 namespace pfh {
 
@@ -59,6 +60,7 @@ template <typename T, int NCOL, int NRB, int LR, int MODE>
 __global__ void __launch_bounds__(NCOL * NRB)
 pf_reg_kernel(BatchDesc d) {
     static_assert((NCOL - LR) % NRB == 0, "register rows must tile the column count exactly");
+    static_assert(NCOL <= 1024, "the pivot key packs the physical column in ten bits");
     constexpr int RB = (NCOL - LR) / NRB;
     using R = typename real_of<T>::type;
     struct XY { T x, y; };
@@ -133,25 +135,31 @@ pf_reg_kernel(BatchDesc d) {
             }
             __syncthreads();   // B1
 
-            // 2. pivot search over logical columns k+1 .. n-1
+            // 2. pivot search over logical columns k+1 .. n-1: every wave
+            //    reduces all columns with DPP (the largest magnitude, then
+            //    the smallest key among the lanes holding it); the key is
+            //    logical index * 1024 + physical column, so ties go to the
+            //    lowest logical index and the column needs no lookup
             double best = -1.0;
-            int bl = 0x7fffffff;
+            int bk = 0x7fffffff;
             for (int cc = lane; cc < NCOL; cc += 64) {
                 const int l = lab[cc];
                 if (l > k && l < n) {
                     const double m = (double)mag2(Xs[cc]);
-                    if (m > best || (m == best && l < bl)) { best = m; bl = l; }
+                    const int key = l * 1024 + cc;
+                    if (m > best || (m == best && key < bk)) { best = m; bk = key; }
                 }
             }
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const double ob = __shfl_xor(best, o);
-                const int ol = __shfl_xor(bl, o);
-                if (ob > best || (ob == best && ol < bl)) { best = ob; bl = ol; }
+            const double wb = wave_max(best);                                  // inputs >= -1, no NaN
+            const int wk = wave_min((best == wb && wb >= 0.0) ? bk : 0x7fffffff);
+            int lq, q;
+            if (wk == 0x7fffffff) {                // only NaN candidates: keep k+1
+                lq = k + 1;
+                q = __builtin_amdgcn_readfirstlane(pos[k + 1]);
+            } else {
+                lq = wk >> 10;
+                q = wk & 1023;
             }
-            int lq = __builtin_amdgcn_readfirstlane(bl);
-            if (lq >= n) lq = k + 1;              // only NaN candidates: keep k+1
-            const int q = __builtin_amdgcn_readfirstlane(pos[lq]);
             const T piv = Xs[q];
             if (is_zero(piv)) { dead = true; break; }
             const bool sw = (lq != k + 1);
