@@ -128,7 +128,18 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
         const int64_t cd = *d.count_dev;
         count = cd < count ? cd : count;
     }
-    for (int64_t jm = blockIdx.x; jm < count; jm += gridDim.x) {
+    // Matrices in the order given (a bucket's list is largest first): the
+    // first by block index, then, with a queue, the next unclaimed one (the
+    // longest-first greedy schedule), else a fixed stride.
+    __shared__ int64_t s_next;
+    auto next = [&](int64_t jm) -> int64_t {
+        if (!d.queue) return jm + gridDim.x;
+        __syncthreads();
+        if (t == 0) s_next = (int64_t)gridDim.x + atomicAdd(d.queue, 1);
+        __syncthreads();
+        return s_next;
+    };
+    for (int64_t jm = blockIdx.x; jm < count; jm = next(jm)) {
         const int idx = d.order ? d.order[(d.begin_dev ? *d.begin_dev : 0) + jm] : (int)jm;
         const int n = __builtin_amdgcn_readfirstlane(d.sizes ? d.sizes[idx] : d.dense_n);
         const int64_t off = d.offsets ? d.offsets[idx] : (int64_t)idx * n * n;

@@ -27,6 +27,8 @@ struct BatchDesc {
     int64_t slab_elems;       //   (lds_spill_elems() doubles each), nullptr when absent
     double* bslab;            // f64 / c128 slog, batched blocked kernel: slab pairs of the resident
     int64_t bslab_elems;      //   workgroups (doubles available), nullptr when absent
+    int32_t* queue;           // optional work counter, zero before the launch: past its first matrix
+                              //   a workgroup takes the next unclaimed one (nullptr: a fixed stride)
 };
 
 enum { MODE_PROD = 0, MODE_SLOG = 1, MODE_GRAD = 2 };

@@ -69,8 +69,11 @@ int resolve_ws(void* ws, size_t ws_bytes, size_t need, void** out) {
 // ---------------------------------------------------------------------------
 constexpr int kMaxBuckets = 8;
 
-__global__ void rg_zero(int32_t* hist, int nbins) {
+// zero the size histogram and the buckets' work counters (a kernel, not a
+// memset, so that a captured graph clears them on every replay)
+__global__ void rg_zero(int32_t* hist, int nbins, int32_t* queue, int nq) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nbins; i += gridDim.x * blockDim.x) hist[i] = 0;
+    if (blockIdx.x == 0 && (int)threadIdx.x < nq) queue[threadIdx.x] = 0;
 }
 
 __global__ void rg_hist(const int32_t* sizes, int64_t count, int32_t max_n, int32_t* hist) {
@@ -150,7 +153,7 @@ int bucket_edges(int dt, int32_t* e) {
 }
 
 struct RaggedLayout {
-    size_t hist, cursor, order, begin, cnt, edges, spill, bat, slabs, total;
+    size_t hist, cursor, order, begin, cnt, edges, queue, spill, bat, slabs, total;
     int glb_grid;
     int64_t slab_elems;
 };
@@ -171,6 +174,7 @@ RaggedLayout ragged_layout(int dt, int64_t count, int32_t max_n) {
     L.begin = off; off = align_up(off + sizeof(int32_t) * (kMaxBuckets + 1));
     L.cnt = off; off = align_up(off + sizeof(int32_t) * (kMaxBuckets + 1));
     L.edges = off; off = align_up(off + sizeof(int32_t) * (kMaxBuckets + 1));
+    L.queue = off; off = align_up(off + sizeof(int32_t) * (kMaxBuckets + 1));
     L.spill = off;
     if (dt == PFH_F64 && max_n > lds_max_n())   // spill slabs of the LDS kernel's 192 < n <= 256 bucket
         off = align_up(off + sizeof(double) * (size_t)lds_spill_elems() * lds_spill_slabs());
@@ -203,6 +207,7 @@ int ragged_typed(int dt, const void* data, const int64_t* offsets, const int32_t
     int32_t* bbegin = (int32_t*)(ws + L.begin);
     int32_t* bcount = (int32_t*)(ws + L.cnt);
     int32_t* edges = (int32_t*)(ws + L.edges);
+    int32_t* queue = (int32_t*)(ws + L.queue);
 
     int32_t e[kMaxBuckets];
     int nb = bucket_edges(dt, e);
@@ -211,7 +216,7 @@ int ragged_typed(int dt, const void* data, const int64_t* offsets, const int32_t
     if (max_n > rmax) e[ne++] = max_n;   // the global-slab bucket
     for (int i = ne; i < 6; ++i) e[i] = e[ne - 1];
     hipLaunchKernelGGL(rg_set_edges, dim3(1), dim3(64), 0, s, edges, e[0], e[1], e[2], e[3], e[4], e[5]);
-    hipLaunchKernelGGL(rg_zero, dim3(1), dim3(256), 0, s, hist, max_n + 1);
+    hipLaunchKernelGGL(rg_zero, dim3(1), dim3(256), 0, s, hist, max_n + 1, queue, kMaxBuckets + 1);
     const int g = (int)((count + 255) / 256 > 1024 ? 1024 : (count + 255) / 256);
     hipLaunchKernelGGL(rg_hist, dim3(g), dim3(256), 0, s, sizes, count, max_n, hist);
     hipLaunchKernelGGL(rg_scan, dim3(1), dim3(256), 0, s, hist, max_n, edges, ne, cursor, bbegin, bcount);
@@ -236,6 +241,7 @@ int ragged_typed(int dt, const void* data, const int64_t* offsets, const int32_t
         if ((dt == PFH_F64 || dt == PFH_C128) && max_n > 64) {
             d.bslab = (double*)(ws + L.bat);
             d.bslab_elems = bat_ws_elems(dt, max_n, count);
+            d.queue = queue + b;            // used by the batched blocked kernel
         }
         if (b >= nb) {
             err = launch_glb<T>(d, (T*)(ws + L.slabs), L.slab_elems, L.glb_grid, s);
