@@ -399,10 +399,10 @@ def test_grad_autograd_and_finite_differences(pf, cuda):
 
 @pytest.mark.parametrize("blocked", ["1", "0"])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
-@pytest.mark.parametrize("n", [520, 1000])
+@pytest.mark.parametrize("n", [520, 530, 1000])
 def test_grad_large_sweeps_vs_inverse(pf, cuda, monkeypatch, dtype, n, blocked):
-    # The blocked block sweep (groups of 32 pairs, 16 for complex128; at
-    # n = 1000 the last group is partial) and the unblocked one
+    # The blocked block sweep (groups of 32 pairs, 16 for complex128; the last
+    # group is partial, of 4, 9 and 20 / 4 pairs) and the unblocked one
     # (PFH_GRAD_BLOCKED=0) against numpy's inverse and the oracle's slog.
     from pfcuda_amd.adjoint import slog_grad
     monkeypatch.setenv("PFH_GRAD_BLOCKED", blocked)
