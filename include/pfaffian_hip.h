@@ -107,6 +107,33 @@ int pfh_slog_pfaffian_grad(int dtype, const void* A, int64_t batch, int32_t n, d
                            void* phase_out, void* grad_out, void* workspace, size_t workspace_bytes,
                            hipStream_t stream);
 
+/* Tuning.  Each key names a choice the library makes at run time.  Its
+ * built-in value was measured on MI355X.  A PFH_* environment variable can
+ * override it, and a value set here overrides both for the rest of the
+ * process; -1 restores the built-in choice.  The autotuner of the host package
+ * (pfcuda_amd/autotune.py) times the candidates on the current device and
+ * sets these.  No counterpart in the reference, whose launch shapes are
+ * fixed at compile time. */
+#define PFH_TUNE_F64_BUCKET_128 0   /* f64 slog, 64 < n <= 128: 0 column-per-lane, 2 LDS-resident, 3 batched blocked */
+#define PFH_TUNE_F64_BUCKET_192 1   /* f64 slog, 128 < n <= 192: same values */
+#define PFH_TUNE_F64_BUCKET_256 2   /* f64 slog, 192 < n <= 256: same values */
+#define PFH_TUNE_C128_BUCKET_128 3  /* c128 slog, 64 < n <= 128: 0 column-per-lane, 3 batched blocked */
+#define PFH_TUNE_TRAIL_F64 4        /* blocked factorization, f64 trailing update: 0 register form, 1 LDS-staged */
+#define PFH_TUNE_TRAIL_C128 5       /* blocked factorization, c128 trailing update: same values */
+#define PFH_TUNE_TRAIL_SWEEP 6      /* block sweep (gradient) trailing update: same values */
+#define PFH_TUNE_PANEL_LAUNCH 7     /* blocked panel: 0 plain launch, 1 cooperative launch */
+#define PFH_TUNE_GRAD_BLOCKED 8     /* gradient past the register kernel: 0 unblocked sweep, 1 blocked sweep */
+#define PFH_TUNE_NKEYS 9
+
+/* Sets a tuning key (value -1: the built-in choice); PFH_EINVAL for an
+ * unknown key or a value outside the key's list.  Takes effect for calls
+ * enqueued afterwards; not for use while another thread enqueues work. */
+int pfh_set_tuning(int key, int value);
+
+/* The value set for a key, or -1 when none is (the environment or the
+ * built-in choice decides). */
+int pfh_get_tuning(int key, int* value);
+
 #ifdef __cplusplus
 }
 #endif

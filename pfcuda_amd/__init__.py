@@ -4,11 +4,22 @@ Public surface (mirrors the reference package pfcuda, pfcuda/__init__.py:1-5):
   pfaffian, slog_pfaffian             -- as pfcuda.pfaffian / pfcuda.slog_pfaffian
   slog_pfaffian_batched               -- slog without the n > 32 guard
   slog_pfaffian_ragged, pack_ragged   -- uneven batches in one call
+  autotune                            -- device tuning of the run-time choices (python -m pfcuda_amd.autotune)
 Importing this package loads libpfaffian_hip.so and fails if it is absent.
 """
+import os as _os
+
 from ._lib import PFH_METHOD_AUTO, PFH_METHOD_BLOCKED, PFH_METHOD_MIXED, PFH_METHOD_UNBLOCKED
 from .api import (PFAFFIAN_MAX_N, pack_ragged, pfaffian, slog_pfaffian, slog_pfaffian_batched,
                   slog_pfaffian_ragged)
+
+# A tuning cache written by pfcuda_amd.autotune, applied when PFH_TUNE_FILE
+# names one (and a GPU is present).
+if _os.environ.get("PFH_TUNE_FILE"):
+    import torch as _torch
+    if _torch.cuda.is_available():
+        from .autotune import load as _load_tuning
+        _load_tuning(_os.environ["PFH_TUNE_FILE"])
 
 __all__ = [
     "pfaffian", "slog_pfaffian", "slog_pfaffian_batched", "slog_pfaffian_ragged", "pack_ragged",

@@ -8,11 +8,21 @@ from __future__ import annotations
 import ctypes
 import os
 
+# torch first: it brings its own HIP runtime, and the library must bind to
+# that one instance (loaded before torch, the library pulls in a second
+# runtime, which then sees no device)
+import torch  # noqa: F401
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpfaffian_hip.so")
 
 PFH_F32, PFH_F64, PFH_C64, PFH_C128 = 0, 1, 2, 3
 PFH_METHOD_AUTO, PFH_METHOD_UNBLOCKED, PFH_METHOD_BLOCKED, PFH_METHOD_MIXED = 0, 1, 2, 3
+# tuning keys (include/pfaffian_hip.h, PFH_TUNE_*)
+TUNE_KEYS = {
+    "f64_bucket_128": 0, "f64_bucket_192": 1, "f64_bucket_256": 2, "c128_bucket_128": 3,
+    "trail_f64": 4, "trail_c128": 5, "trail_sweep": 6, "panel_launch": 7, "grad_blocked": 8,
+}
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -43,6 +53,8 @@ def _load():
     lib.pfh_slog_pfaffian_ragged.argtypes = [ctypes.c_int, _vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _sz, _vp]
     lib.pfh_slog_pfaffian_grad_workspace_size.argtypes = [ctypes.c_int, _i64, _i32, ctypes.POINTER(_sz)]
     lib.pfh_slog_pfaffian_grad.argtypes = [ctypes.c_int, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _sz, _vp]
+    lib.pfh_set_tuning.argtypes = [ctypes.c_int, ctypes.c_int]
+    lib.pfh_get_tuning.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     return lib
 
 

@@ -32,6 +32,9 @@ struct BatchDesc {
 };
 
 enum { MODE_PROD = 0, MODE_SLOG = 1, MODE_GRAD = 2 };
+
+// The value set with pfh_set_tuning for a key (PFH_TUNE_*), or -1.
+int tune_value(int key);
 enum { DT_F32 = 0, DT_F64 = 1, DT_C64 = 2, DT_C128 = 3 };
 
 // Largest n the register-resident workgroup kernels take for a dtype

@@ -30,6 +30,7 @@
 #include <cstdlib>
 #include <type_traits>
 
+#include "../../include/pfaffian_hip.h"
 #include "pf_batch.h"
 #include "pf_blocked.h"
 #include "pf_types.h"
@@ -934,7 +935,9 @@ __global__ void __launch_bounds__(256) bp_trailing_staged(const T* __restrict__ 
 // (f64 N = 2048: 8.0 / 8.6, N = 4096: 18.5 / 22.5; c128 N = 4096: 30.5 /
 // 32.1) use the staged form.  PFH_TRAIL_STAGED=1 or 0 forces one form
 // everywhere (read per call).
-static bool trail_staged(bool dflt) {
+static bool trail_staged(int key, bool dflt) {
+    const int tv = tune_value(key);
+    if (tv >= 0) return tv == 1;
     const char* e = std::getenv("PFH_TRAIL_STAGED");
     if (e && e[0] == '0') return false;
     if (e && e[0] == '1') return true;
@@ -1025,8 +1028,10 @@ static hipError_t launch_panel(const BPArgs<T>& a, const BPCoop<T>& co, int k0, 
     // ~15 us per panel in a fresh process and ~100 us per panel in one that
     // has captured and replayed graphs on several streams (N = 4096 f64:
     // 12.6 ms plain, 13.5 / 20.5 ms cooperative).
+    const int tv = tune_value(PFH_TUNE_PANEL_LAUNCH);
     const char* lm = std::getenv("PFH_PANEL_LAUNCH");
-    if (!(lm && lm[0] == 'c')) {
+    const bool coop = tv >= 0 ? tv == 1 : (lm && lm[0] == 'c');
+    if (!coop) {
         hipLaunchKernelGGL((bp_panel<T, NB>), dim3(G), dim3(BC_T), smem, s, a, co, k0, np, epoch0);
         return hipGetLastError();
     }
@@ -1114,7 +1119,7 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
                 if (e != hipSuccess) return (int)e;
             }
             if constexpr (std::is_same<T, double>::value) {
-                if (nb == 32 && trail_staged(false)) {
+                if (nb == 32 && trail_staged(PFH_TUNE_TRAIL_F64, false)) {
                     hipLaunchKernelGGL((bp_trailing_staged<double, 32>), dim3(tiles), dim3(256), 0, s, W[cur],
                                        W[cur ^ 1], a.P, a.perm, n, (int)lo, np, ntile, panel_coop ? 1 : 0);
                 } else if (nb == 32) {
@@ -1125,7 +1130,7 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
                                        a.perm, n, (int)lo, nb, np, ntile, panel_coop ? 1 : 0);
                 }
             } else if constexpr (std::is_same<T, c128>::value) {
-                if (nb == 16 && trail_staged(true)) {
+                if (nb == 16 && trail_staged(PFH_TUNE_TRAIL_C128, true)) {
                     hipLaunchKernelGGL((bp_trailing_staged<c128, 16>), dim3(tiles), dim3(256), 0, s, W[cur],
                                        W[cur ^ 1], a.P, a.perm, n, (int)lo, np, ntile, panel_coop ? 1 : 0);
                 } else if (nb == 16) {
@@ -1581,8 +1586,9 @@ static GJLayout<T> gj_layout(int64_t n, int nb) {
 // The blocked sweep needs the cooperative panel over the whole first
 // block; PFH_GRAD_BLOCKED=0 selects the unblocked sweep (read per call).
 bool blocked_grad_enabled(int64_t n) {
+    const int tv = tune_value(PFH_TUNE_GRAD_BLOCKED);
     const char* e = std::getenv("PFH_GRAD_BLOCKED");
-    if (e && e[0] == '0') return false;
+    if (tv == 0 || (tv < 0 && e && e[0] == '0')) return false;
     return coop_enabled() && (n + BC_T - 1) / BC_T <= BC_MAXG && n % 2 == 0;
 }
 
@@ -1641,7 +1647,7 @@ int blocked_grad(const T* A, int64_t n, void* ws, double* log_out, T* phase_out,
         hipLaunchKernelGGL((gj_rows<T, NB>), dim3((unsigned)((n + 31) / 32)), dim3(256), 0, s, W[cur], a.perm, n,
                            (int)k0, np, a.st, Ai, S, Z, gp[gc], gp[gc ^ 1]);
         if constexpr (std::is_same<T, double>::value || std::is_same<T, c128>::value) {
-            if (trail_staged(true)) {
+            if (trail_staged(PFH_TUNE_TRAIL_SWEEP, true)) {
                 hipLaunchKernelGGL((gj_trailing_staged<T, 2 * NB>), dim3(tiles), dim3(256), 0, s, W[cur], W[cur ^ 1],
                                    S, Z, a.perm, n, (int)k0, m, ntile, a.st);
                 cur ^= 1;

@@ -1,5 +1,6 @@
 // C-ABI entry points (declared in include/pfaffian_hip.h) and the
 // device-side size bucketing of ragged batches.
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -23,6 +24,25 @@ size_t dtype_bytes(int dt) {
 }
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Tuning keys set through the C ABI, stored as value + 1 (0: not set).
+std::atomic<int> g_tune[PFH_TUNE_NKEYS];
+
+bool tune_value_ok(int key, int v) {
+    if (v == -1) return true;
+    switch (key) {
+        case PFH_TUNE_F64_BUCKET_128:
+        case PFH_TUNE_F64_BUCKET_192:
+        case PFH_TUNE_F64_BUCKET_256: return v == 0 || v == 2 || v == 3;
+        case PFH_TUNE_C128_BUCKET_128: return v == 0 || v == 3;
+        case PFH_TUNE_TRAIL_F64:
+        case PFH_TUNE_TRAIL_C128:
+        case PFH_TUNE_TRAIL_SWEEP:
+        case PFH_TUNE_PANEL_LAUNCH:
+        case PFH_TUNE_GRAD_BLOCKED: return v == 0 || v == 1;
+    }
+    return false;
+}
 
 // Cached per-device workspace for callers that pass none.
 struct WsCache {
@@ -391,7 +411,24 @@ int grad_dense(int dt, const T* A, int64_t batch, int32_t n, double* log_out, T*
 
 }  // namespace
 
+int pfh::tune_value(int key) {
+    if (key < 0 || key >= PFH_TUNE_NKEYS) return -1;
+    return g_tune[key].load(std::memory_order_relaxed) - 1;
+}
+
 extern "C" {
+
+int pfh_set_tuning(int key, int value) {
+    if (key < 0 || key >= PFH_TUNE_NKEYS || !tune_value_ok(key, value)) return PFH_EINVAL;
+    g_tune[key].store(value + 1, std::memory_order_relaxed);
+    return 0;
+}
+
+int pfh_get_tuning(int key, int* value) {
+    if (key < 0 || key >= PFH_TUNE_NKEYS || !value) return PFH_EINVAL;
+    *value = tune_value(key);
+    return 0;
+}
 
 int pfh_slog_pfaffian_grad_workspace_size(int dtype, int64_t batch, int32_t n, size_t* bytes) {
     if (!bytes || dtype_bytes(dtype) == 0 || batch < 0 || n < 0) return PFH_EINVAL;

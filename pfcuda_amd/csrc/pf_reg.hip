@@ -45,6 +45,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "../../include/pfaffian_hip.h"
 #include "pf_batch.h"
 #include "pf_regstore.h"
 #include "pf_wave.h"
@@ -390,7 +391,10 @@ static int f64_kernel(int mode, int maxn, const BatchDesc& d) {
     if (mode == MODE_GRAD) return 0;
     if (mode == MODE_PROD && maxn > 64) return 0;
     if (maxn > 256) return 0;
-    const int o = f64_kernel_override();
+    const int key = maxn <= 64 ? -1 : maxn <= 128 ? PFH_TUNE_F64_BUCKET_128 : maxn <= 192 ? PFH_TUNE_F64_BUCKET_192
+                                                                            : PFH_TUNE_F64_BUCKET_256;
+    const int tv = (mode == MODE_SLOG && key >= 0) ? tune_value(key) : -1;
+    const int o = tv >= 0 ? tv : f64_kernel_override();
     const bool bat_ok = mode == MODE_SLOG && maxn > 64 && d.bslab;
     if (o == 3) return bat_ok ? 3 : 0;
     if (o < 0 && bat_ok) return 3;
@@ -409,11 +413,13 @@ int launch_reg(int dtype, int mode, const BatchDesc& d, int maxn, hipStream_t s)
     if (dtype == DT_C128 && mode == MODE_SLOG && maxn > 64) {
         // complex128 slog: the batched blocked kernel above 64 (the lanes
         // kernel holds at most 128; PFH_C128_KERNEL=lanes keeps it there)
-        static int lanes = -1;
-        if (lanes < 0) {
+        static int lanes_env = -1;
+        if (lanes_env < 0) {
             const char* e = getenv("PFH_C128_KERNEL");
-            lanes = (e && strcmp(e, "lanes") == 0) ? 1 : 0;
+            lanes_env = (e && strcmp(e, "lanes") == 0) ? 1 : 0;
         }
+        const int tv = maxn <= 128 ? tune_value(PFH_TUNE_C128_BUCKET_128) : -1;
+        const bool lanes = tv >= 0 ? tv == 0 : lanes_env == 1;
         if (d.bslab && !(lanes && maxn <= 128)) return launch_bat(DT_C128, d, maxn, s);
         if (maxn > 128) return (int)hipErrorInvalidValue;
     }

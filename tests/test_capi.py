@@ -83,3 +83,48 @@ def test_host_guards_without_gpu():
         pfcuda_amd.pfaffian(np.zeros(3))
     with pytest.raises(ValueError):
         pfcuda_amd.pfaffian(np.zeros((3, 4)))
+
+
+def test_tuning_keys_match_header():
+    from pfcuda_amd import _lib
+    text = open(HEADER).read()
+    keys = {m.group(1).lower(): int(m.group(2))
+            for m in re.finditer(r"^#define PFH_TUNE_(\w+) (\d+)", text, re.M) if m.group(1) != "NKEYS"}
+    assert keys == _lib.TUNE_KEYS
+    assert int(re.search(r"^#define PFH_TUNE_NKEYS (\d+)", text, re.M).group(1)) == len(keys)
+
+
+def test_set_get_tuning(lib):
+    # host-only: values are checked against each key's list, -1 restores
+    # the built-in choice; nothing here touches a GPU
+    v = ctypes.c_int(7)
+    try:
+        assert lib.pfh_get_tuning(4, ctypes.byref(v)) == 0 and v.value == -1
+        assert lib.pfh_set_tuning(4, 1) == 0
+        assert lib.pfh_get_tuning(4, ctypes.byref(v)) == 0 and v.value == 1
+        assert lib.pfh_set_tuning(4, 2) == -1            # not a value of this key
+        assert lib.pfh_set_tuning(0, 1) == -1            # 1 is no kernel of an f64 bucket
+        assert lib.pfh_set_tuning(0, 2) == 0
+        assert lib.pfh_set_tuning(9, 0) == -1 and lib.pfh_set_tuning(-1, 0) == -1
+        assert lib.pfh_get_tuning(9, ctypes.byref(v)) == -1
+        assert lib.pfh_get_tuning(0, None) == -1
+    finally:
+        for k in range(9):
+            lib.pfh_set_tuning(k, -1)
+
+
+def test_autotune_cache_applies_for_its_architecture_only(tmp_path):
+    import json
+    from pfcuda_amd import autotune
+    path = tmp_path / "tune.json"
+    path.write_text(json.dumps({"arch": "gfxTEST", "choices": {"trail_f64": 1, "f64_bucket_192": 2}}))
+    try:
+        assert autotune.load(str(path), arch="gfxOTHER") == {}
+        assert autotune.get_tuning("trail_f64") == -1
+        assert autotune.load(str(path), arch="gfxTEST") == {"trail_f64": 1, "f64_bucket_192": 2}
+        assert autotune.get_tuning("trail_f64") == 1 and autotune.get_tuning("f64_bucket_192") == 2
+        with pytest.raises(KeyError):
+            autotune.apply({"no_such_key": 0})
+    finally:
+        for name in autotune.CANDIDATES:
+            autotune.set_tuning(name, -1)
