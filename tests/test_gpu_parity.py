@@ -501,3 +501,61 @@ def test_grad_n2048_residual(pf, cuda):
     assert rel < 1e-13, rel
     l0, p0 = oracle.slog(A)
     assert abs(float(la) - l0) < 1e-8 and float(ph) == p0
+
+
+@pytest.mark.parametrize("key,lo,hi,values,cplx", [
+    ("f64_bucket_128", 64, 128, (0, 2, 3), False),
+    ("f64_bucket_192", 128, 192, (0, 2, 3), False),
+    ("f64_bucket_256", 192, 256, (0, 2, 3), False),
+    ("c128_bucket_128", 64, 128, (0, 3), True),
+])
+def test_every_bucket_kernel_vs_oracle(pf, cuda, key, lo, hi, values, cplx):
+    # Each kernel a size bucket can run (column per lane, LDS-resident with
+    # its spill slab above 192, batched blocked), chosen through the tuning
+    # key, on a ragged batch of that bucket: the oracle, and the same
+    # results whichever kernel runs.
+    from pfcuda_amd import autotune
+    rng = np.random.default_rng(hi + (7 if cplx else 0))
+    sizes = [int(s) for s in 2 * rng.integers(lo // 2 + 1, hi // 2 + 1, size=24)] + [hi, lo + 2]
+    mats = [oracle.skew_gaussian(rng, n, complex_=cplx) for n in sizes]
+    data, offsets, sz = pf.pack_ragged([torch.from_numpy(m) for m in mats])
+    want = [oracle.slog(m) for m in mats]
+    try:
+        for v in values:
+            autotune.set_tuning(key, v)
+            la, ph = pf.slog_pfaffian_ragged(data.to(cuda), offsets, sz)
+            la, ph = la.cpu().numpy(), ph.cpu().numpy()
+            for i, (l0, p0) in enumerate(want):
+                assert abs(la[i] - l0) < 1e-9 and abs(complex(ph[i]) - p0) < 1e-9, (key, v, sizes[i])
+    finally:
+        autotune.set_tuning(key, -1)
+
+
+def test_per_step_blocked_path_and_unblocked_sweep(cuda):
+    # With the panel kernel off (PFH_BLOCKED_COOP=0, read once per process,
+    # hence a child process): the blocked factorization launches once per
+    # pair step and the large gradient uses the unblocked sweep; both against
+    # the oracle and numpy's inverse.
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script = r"""
+import sys, numpy as np, torch
+sys.path.insert(0, ROOT); sys.path.insert(0, ROOT + "/tests")
+import oracle, pfcuda_amd as pf
+from pfcuda_amd.adjoint import slog_grad
+rng = np.random.default_rng(91)
+for cplx in (False, True):
+    A = oracle.skew_gaussian(rng, 600, complex_=cplx)
+    la, s = pf.slog_pfaffian(torch.from_numpy(A).cuda(), method=pf.PFH_METHOD_BLOCKED)
+    l0, p0 = oracle.slog(A)
+    assert abs(float(la) - l0) < 1e-9 and abs(complex(s.item()) - p0) < 1e-9, (cplx, float(la), l0)
+    B = oracle.skew_gaussian(rng, 300, complex_=cplx)
+    G = slog_grad(torch.from_numpy(B).cuda()).cpu().numpy()
+    want = 0.5 * np.linalg.inv(B).T
+    assert np.abs(G - want).max() / np.abs(want).max() < 1e-10
+print("per-step ok", flush=True)
+""".replace("ROOT", repr(root))
+    env = dict(os.environ, PFH_BLOCKED_COOP="0")
+    r = subprocess.run([sys.executable, "-c", script], env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "per-step ok" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
