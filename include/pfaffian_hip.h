@@ -107,6 +107,14 @@ int pfh_slog_pfaffian_grad(int dtype, const void* A, int64_t batch, int32_t n, d
                            void* phase_out, void* grad_out, void* workspace, size_t workspace_bytes,
                            hipStream_t stream);
 
+/* Pfaffian of one row-major n x n float64 matrix in HOST memory, on the
+ * CPU (FP64, one thread, product form), working in place on A: the
+ * counterpart of the reference's public pfaffian_cpu (pfcuda/cpp_api.py:3-4,
+ * bindings/pybind_bindings.cpp:8-30, src/pfaffian_cpu.cpp:7-106).  n = 0
+ * gives 0, n = 2 and 4 the closed forms; odd n returns PFH_EINVAL.  A
+ * separate host entry point: no GPU path calls it. */
+int pfh_pfaffian_cpu_f64(double* A, int32_t n, double* out);
+
 /* Tuning.  Each key names a choice the library makes at run time.  Its
  * built-in value was measured on MI355X.  A PFH_* environment variable can
  * override it, and a value set here overrides both for the rest of the

@@ -2,6 +2,7 @@
 
 Public surface (mirrors the reference package pfcuda, pfcuda/__init__.py:1-5):
   pfaffian, slog_pfaffian             -- as pfcuda.pfaffian / pfcuda.slog_pfaffian
+  pfaffian_cpu, pfaffian_py           -- as pfcuda.pfaffian_cpu / pfcuda.pfaffian_py (host data, CPU)
   slog_pfaffian_batched               -- slog without the n > 32 guard
   slog_pfaffian_ragged, pack_ragged   -- uneven batches in one call
   autotune                            -- device tuning of the run-time choices (python -m pfcuda_amd.autotune)
@@ -12,6 +13,7 @@ import os as _os
 from ._lib import PFH_METHOD_AUTO, PFH_METHOD_BLOCKED, PFH_METHOD_MIXED, PFH_METHOD_UNBLOCKED
 from .api import (PFAFFIAN_MAX_N, pack_ragged, pfaffian, slog_pfaffian, slog_pfaffian_batched,
                   slog_pfaffian_ragged)
+from .cpu import pfaffian_cpu, pfaffian_py
 
 # A tuning cache written by pfcuda_amd.autotune, applied when PFH_TUNE_FILE
 # names one (and a GPU is present).
@@ -22,7 +24,8 @@ if _os.environ.get("PFH_TUNE_FILE"):
         _load_tuning(_os.environ["PFH_TUNE_FILE"])
 
 __all__ = [
-    "pfaffian", "slog_pfaffian", "slog_pfaffian_batched", "slog_pfaffian_ragged", "pack_ragged",
+    "pfaffian", "slog_pfaffian", "pfaffian_cpu", "pfaffian_py",
+    "slog_pfaffian_batched", "slog_pfaffian_ragged", "pack_ragged",
     "PFAFFIAN_MAX_N", "PFH_METHOD_AUTO", "PFH_METHOD_UNBLOCKED", "PFH_METHOD_BLOCKED",
     "PFH_METHOD_MIXED",
 ]
