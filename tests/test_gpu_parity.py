@@ -559,3 +559,20 @@ print("per-step ok", flush=True)
     env = dict(os.environ, PFH_BLOCKED_COOP="0")
     r = subprocess.run([sys.executable, "-c", script], env=env, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and "per-step ok" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+def test_blocked_panel_wider_than_the_panel_kernel(pf, cuda):
+    # n = 16448: the first panel spans 65 blocks of 256 columns, one more than
+    # the panel kernel takes, so it runs as per-step launches, and the panels
+    # after it on the panel kernel.  Against the unblocked path (independent
+    # kernels) on the same matrix; the oracle would take minutes here.
+    n = 16448
+    g = torch.Generator(device=cuda)
+    g.manual_seed(16448)
+    X = torch.randn(n, n, generator=g, device=cuda, dtype=torch.float64)
+    A = ((X - X.T) / 2).contiguous()
+    del X
+    lb, sb = pf.slog_pfaffian(A, method=pf.PFH_METHOD_BLOCKED)
+    lu, su = pf.slog_pfaffian(A, method=pf.PFH_METHOD_UNBLOCKED)
+    assert math.isfinite(float(lb)) and abs(float(lb) - float(lu)) < 1e-8, (float(lb), float(lu))
+    assert float(sb) == float(su)
