@@ -62,7 +62,7 @@ int cached_workspace(size_t need, void** out) {
         if (g_ws.ptr[dev]) {
             e = hipDeviceSynchronize();
             if (e != hipSuccess) return (int)e;
-            hipFree(g_ws.ptr[dev]);
+            (void)hipFree(g_ws.ptr[dev]);
             g_ws.ptr[dev] = nullptr;
             g_ws.bytes[dev] = 0;
         }
@@ -180,7 +180,7 @@ struct RaggedLayout {
 
 int glb_grid() {
     int dev = 0, v = 0;
-    hipGetDevice(&dev);
+    (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
     return 2 * v;
 }

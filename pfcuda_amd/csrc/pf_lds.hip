@@ -401,7 +401,7 @@ int lds_num_cu() {
     static int v = 0;
     if (v == 0) {
         int dev = 0;
-        hipGetDevice(&dev);
+        (void)hipGetDevice(&dev);
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
     }
     return v;

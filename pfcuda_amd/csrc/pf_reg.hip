@@ -256,7 +256,7 @@ static int g_num_cu = 0;
 static int num_cu() {
     if (g_num_cu == 0) {
         int dev = 0;
-        hipGetDevice(&dev);
+        (void)hipGetDevice(&dev);
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
         g_num_cu = v;
