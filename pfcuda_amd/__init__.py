@@ -6,6 +6,8 @@ Public surface (mirrors the reference package pfcuda, pfcuda/__init__.py:1-5):
   slog_pfaffian_batched               -- slog without the n > 32 guard
   slog_pfaffian_ragged, pack_ragged   -- uneven batches in one call
   autotune                            -- device tuning of the run-time choices (python -m pfcuda_amd.autotune)
+  torch.ops.pfcuda.{pfaffian, slog_pfaffian, slog_pfaffian_grad, slog_pfaffian_ragged}
+                                      -- the entry points as torch.library operators (compile, vmap, grad)
 Importing this package loads libpfaffian_hip.so and fails if it is absent.
 """
 import os as _os
@@ -13,6 +15,7 @@ import os as _os
 from ._lib import PFH_METHOD_AUTO, PFH_METHOD_BLOCKED, PFH_METHOD_MIXED, PFH_METHOD_UNBLOCKED
 from .api import (PFAFFIAN_MAX_N, pack_ragged, pfaffian, slog_pfaffian, slog_pfaffian_batched,
                   slog_pfaffian_ragged)
+from .adjoint import slog_grad  # noqa: F401  (also registers pfcuda::slog_pfaffian_grad)
 from .cpu import pfaffian_cpu, pfaffian_py
 
 # A tuning cache written by pfcuda_amd.autotune, applied when PFH_TUNE_FILE

@@ -14,7 +14,9 @@ through ``torch.autograd.Function`` with both ``jvp`` (forward mode) and
   d log|Pf| = 1/2 tr(A^-1 dA)          ->  A_bar = g * 1/2 A^-T
 For complex inputs the log-magnitude tangent is the real part of the trace
 and the phase tangent is i*Im(...)*phase (the reference returns a complex
-log tangent and a zero phase tangent, cuda_api.py:103-104).
+log tangent and a zero phase tangent, cuda_api.py:103-104); reverse mode
+takes the phase's cotangent as well.  A real input's sign is piecewise
+constant and carries no derivative.
 """
 from __future__ import annotations
 
@@ -119,13 +121,88 @@ def _trace_prod(X: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
 
 
 # ---------------------------------------------------------------------------
+# operators: the entry points as torch.library primitives
+# ---------------------------------------------------------------------------
+# The reference registers each FFI handler with its framework
+# (pfcuda/cuda_api.py:10-18) and calls it through jax.ffi.ffi_call
+# (:47-52, :69-74).  Here each C-ABI entry point is an operator in the
+# "pfcuda" namespace with
+#   - a shape-only (fake) implementation, so that torch.compile traces
+#     through it without a graph break and meta tensors propagate shapes;
+#   - a batching rule that folds a vmapped dimension into the kernels' own
+#     leading batch, so that torch.func.vmap costs one launch, not a loop.
+# Their outputs never alias their inputs and the kernels never write them.
+
+def _front(x: torch.Tensor, d):
+    """x with its vmapped dimension d moved to the front (d is not None)."""
+    return x.movedim(d, 0)
+
+
+@torch.library.custom_op("pfcuda::pfaffian", mutates_args=(), schema="(Tensor A) -> Tensor")
+def _pfaffian_op(A):
+    return _pf_kernel(A)
+
+
+@_pfaffian_op.register_fake
+def _(A):
+    return A.new_empty(A.shape[:-2])
+
+
+def _pfaffian_vmap(info, in_dims, A):
+    (d,) = in_dims
+    if d is None:
+        return _pfaffian_op(A), None
+    return _pfaffian_op(_front(A, d)), 0
+
+
+_pfaffian_op.register_vmap(_pfaffian_vmap)
+
+
+@torch.library.custom_op("pfcuda::slog_pfaffian", mutates_args=(),
+                         schema="(Tensor A, int method) -> (Tensor, Tensor)")
+def _slog_op(A, method):
+    return _slog_kernel(A, method)
+
+
+@_slog_op.register_fake
+def _(A, method):
+    return (A.new_empty(A.shape[:-2], dtype=torch.float64), A.new_empty(A.shape[:-2]))
+
+
+def _slog_vmap(info, in_dims, A, method):
+    d = in_dims[0]
+    if d is None:
+        return _slog_op(A, method), (None, None)
+    return _slog_op(_front(A, d), method), (0, 0)
+
+
+_slog_op.register_vmap(_slog_vmap)
+
+
+@torch.library.custom_op("pfcuda::slog_pfaffian_ragged", mutates_args=(),
+                         schema="(Tensor data, Tensor offsets, Tensor sizes, int max_n) -> (Tensor, Tensor)")
+def _ragged_op(data, offsets, sizes, max_n):
+    return _ragged_kernel(data, offsets, sizes, max_n)
+
+
+@_ragged_op.register_fake
+def _(data, offsets, sizes, max_n):
+    count = sizes.shape[0]
+    return (data.new_empty((count,), dtype=torch.float64), data.new_empty((count,)))
+
+
+# ---------------------------------------------------------------------------
 # pfaffian
 # ---------------------------------------------------------------------------
 
 class _PfaffianFn(torch.autograd.Function):
+    # under vmap the forward runs batched and reaches the operator's
+    # batching rule; backward and jvp are torch operations
+    generate_vmap_rule = True
+
     @staticmethod
     def forward(A):
-        return _pf_kernel(A)
+        return _pfaffian_op(A)
 
     @staticmethod
     def setup_context(ctx, inputs, output):
@@ -177,9 +254,11 @@ def pfaffian(A):
 # ---------------------------------------------------------------------------
 
 class _SlogFn(torch.autograd.Function):
+    generate_vmap_rule = True
+
     @staticmethod
     def forward(A, method):
-        return _slog_kernel(A, method)
+        return _slog_op(A, method)
 
     @staticmethod
     def setup_context(ctx, inputs, output):
@@ -187,16 +266,23 @@ class _SlogFn(torch.autograd.Function):
         logabs, phase = output
         ctx.save_for_backward(A, phase)
         ctx.save_for_forward(A, phase)
-        ctx.mark_non_differentiable(phase)
+        if not A.is_complex():
+            ctx.mark_non_differentiable(phase)    # a sign: piecewise constant
 
     @staticmethod
     def backward(ctx, g_log, g_phase):
-        A, _ = ctx.saved_tensors
+        # With t = 1/2 tr(A^-1 dA) = sum(G * dA), G = 1/2 A^-T:
+        #   d log|Pf| = Re t,  d phase = i Im(t) phase  (complex),
+        # so A_bar = (g_log + i c) conj(G) with c = -Im(conj(g_phase) phase).
+        A, phase = ctx.saved_tensors
         from .adjoint import slog_grad
         G = slog_grad(A)                      # 1/2 A^-T, from the HIP adjoint
-        if A.is_complex():
-            G = G.conj()
-        return g_log.to(G.dtype)[..., None, None] * G, None
+        if not A.is_complex():
+            return g_log.to(G.dtype)[..., None, None] * G, None
+        coef = g_log.to(G.dtype)
+        if g_phase is not None:
+            coef = coef + 1j * (-(g_phase.conj() * phase).imag)
+        return coef[..., None, None] * G.conj(), None
 
     @staticmethod
     def jvp(ctx, dA, _dmethod):
@@ -204,7 +290,7 @@ class _SlogFn(torch.autograd.Function):
         tr = 0.5 * _trace_prod(_inv(A), dA)
         if A.is_complex():
             return tr.real, 1j * tr.imag * phase
-        return tr, torch.zeros_like(phase)
+        return tr, None
 
 
 def _slog_guarded(A, method: int, min_n: int):
@@ -252,21 +338,11 @@ def pack_ragged(mats: Sequence[torch.Tensor]):
     return data, offsets, sizes
 
 
-def slog_pfaffian_ragged(data, offsets, sizes, max_n: int | None = None):
-    """(log|Pf|, sign/phase) of every matrix of a packed ragged batch.
-
-    ``data`` holds the matrices back to back, matrix i being the
-    sizes[i] x sizes[i] row-major slab at element offsets[i].  One call, one
-    persistent launch per size bucket, no host synchronisation.
-    """
-    data = _to_device(data)
+def _ragged_kernel(data: torch.Tensor, offsets: torch.Tensor, sizes: torch.Tensor, max_n: int):
+    """The ragged entry point on device tensors: contiguous data, int64
+    offsets and int32 sizes, all on one GPU."""
     dev = data.device
-    offsets = torch.as_tensor(offsets, dtype=torch.int64).to(dev).contiguous()
-    sizes_t = torch.as_tensor(sizes, dtype=torch.int32)
-    if max_n is None:
-        max_n = int(sizes_t.max().item()) if sizes_t.numel() else 0
-    sizes_t = sizes_t.to(dev).contiguous()
-    count = int(sizes_t.numel())
+    count = int(sizes.numel())
     dt = _DTYPES[data.dtype]
     logabs = torch.empty(count, dtype=torch.float64, device=dev)
     phase = torch.empty(count, dtype=data.dtype, device=dev)
@@ -275,9 +351,26 @@ def slog_pfaffian_ragged(data, offsets, sizes, max_n: int | None = None):
     need = _lib._sz(0)
     check(lib.pfh_slog_pfaffian_ragged_workspace_size(dt, count, int(max_n), need), "workspace query")
     ws = torch.empty(max(int(need.value), 1), dtype=torch.uint8, device=dev)
-    data = data.contiguous()
-    check(lib.pfh_slog_pfaffian_ragged(dt, data.data_ptr(), offsets.data_ptr(), sizes_t.data_ptr(), count,
+    check(lib.pfh_slog_pfaffian_ragged(dt, data.data_ptr(), offsets.data_ptr(), sizes.data_ptr(), count,
                                        int(max_n), logabs.data_ptr(), phase.data_ptr(), ws.data_ptr(),
                                        ws.numel(), _stream(data)),
           "slog_pfaffian_ragged")
     return logabs, phase
+
+
+def slog_pfaffian_ragged(data, offsets, sizes, max_n: int | None = None):
+    """(log|Pf|, sign/phase) of every matrix of a packed ragged batch.
+
+    ``data`` holds the matrices back to back, matrix i being the
+    sizes[i] x sizes[i] row-major slab at element offsets[i].  One call, one
+    persistent launch per size bucket, no host synchronisation once
+    ``max_n`` is given (without it, the largest size is read back first).
+    """
+    data = _to_device(data).contiguous()
+    dev = data.device
+    offsets = torch.as_tensor(offsets, dtype=torch.int64).to(dev).contiguous()
+    sizes_t = torch.as_tensor(sizes, dtype=torch.int32)
+    if max_n is None:
+        max_n = int(sizes_t.max().item()) if sizes_t.numel() else 0
+    sizes_t = sizes_t.to(dev).contiguous()
+    return _ragged_op(data, offsets, sizes_t, int(max_n))
