@@ -43,7 +43,14 @@ constexpr int BP_TILE = 64;        // trailing-update tile edge
 constexpr int BC_T = 256;          // cooperative panel: threads per block = slots per block
 constexpr int BC_MAXG = 64;        // cooperative panel: blocks at most (one wave reduces the partials)
 constexpr int BC_WAVES = BC_T / 64;
-constexpr int BC_CPW = BC_MAXG / BC_WAVES;   // candidate rows held per wave
+// candidate records (described with bc_poll): the top bit carries the
+// epoch's parity; "no candidate" is NaN bits in A and index 2^31 - 1, slot -1
+// in B
+constexpr unsigned long long BC_PAR = 1ull << 63;
+constexpr unsigned long long BC_NONE_A = 0x7ff8000000000001ull;
+constexpr unsigned long long BC_NONE_B = (0x7fffffffull << 32) | 0xffffffffull;
+
+__host__ __device__ constexpr unsigned long long bc_par(int e) { return (e & 1) ? BC_PAR : 0ull; }
 
 template <typename T>
 struct BPState {
@@ -122,14 +129,24 @@ __device__ __forceinline__ void accumulate(BPState<T>* st, T piv, bool sw) {
 }
 
 template <typename T>
-__global__ void bp_init(BPArgs<T> a, int* flags) {
+__global__ void bp_init(BPArgs<T> a, unsigned long long* rec) {
     const int64_t n = a.n;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         a.perm[i] = (int)i;
+    // candidate records: "no candidate", with the parity opposite to that of
+    // the first epoch each buffer serves (epochs count from 1: buffer 1
+    // serves 1, 4, 7, ...; buffer 2 serves 2, 5, ...; buffer 0 serves 3, 6, ...)
+    if (blockIdx.x == 0) {
+        for (int q = threadIdx.x; q < 3 * BC_MAXG; q += blockDim.x) {
+            const int buf = q / BC_MAXG;
+            const unsigned long long pb = bc_par((buf == 0 ? 3 : buf) + 1);
+            rec[2 * q] = BC_NONE_A | pb;
+            rec[2 * q + 1] = BC_NONE_B | pb;
+        }
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         BPState<T>* st = a.st;
         st->p = 0; st->dead = 0; st->counter = 0; st->err = 0;
-        for (int g = 0; g < BC_MAXG; ++g) flags[g] = 0;
         st->piv = one<T>();
         st->lm = 1.0; st->le = 0;
         st->ph_re = 1.0; st->ph_im = 0.0;
@@ -340,28 +357,33 @@ struct alignas(2 * sizeof(T)) BCPair {
     T u, v;
 };
 
-template <typename T>
-struct BPPart {
-    double best;
-    int idx, slot;
-    T val;
-};
+// Candidate records: the grid barrier and the exchange of the partials in
+// one.  A block's record for epoch E (one per pivot row) is two 64-bit words
+// in buffer E % 3, each carrying E's parity in its top bit:
+//   A = the candidate's magnitude, a double >= 0 (NaN bits: no candidate);
+//   B = its logical index (bits 32..62) and slot (bits 0..31).
+// The buffer a block overwrites at epoch E last held its record E - 3, which
+// has the other parity, and every block has finished reading it: a block
+// publishing E has seen every record E - 1, each published after its
+// author's reads of E - 2 and earlier.  So a reader polling for E sees either
+// E - 3 (keeps polling) or E, word by word, and once both words show E, the
+// block's other exchange stores (its candidate's panel entries and value,
+// its panel vectors) have landed: they precede the record's stores.
 
 template <typename T>
 struct BPCoop {
-    BPPart<T>* part;   // [2][BC_MAXG]
-    T* cand;           // [2][BC_MAXG][2*nb]: U_j at 2j, V_j at 2j + 1
-    int* flags;        // [BC_MAXG]: the last barrier epoch each block reached
+    unsigned long long* rec;   // [3][BC_MAXG][2]: records A, B by epoch % 3
+    T* cand;                   // [3][BC_MAXG][2 nb + 2]: the candidate's U_j at 2j, V_j at 2j + 1, its value at 2 nb
 };
 
 // Accesses to the data the panel's blocks exchange within a launch (panel
-// vectors, partials, candidates).  NF (real types): agent-scope relaxed
-// atomics, coherent across the XCDs' L2s on their own, so that the barrier
-// needs no L2 write-back / invalidate, only a wait for the block's stores
-// before its flag (N = 4096 f64 13.5 -> 12.7 ms, N = 8192 33.6 -> 30.9 ms,
-// results bit-identical).  Complex types keep plain accesses and the
-// barrier's agent-scope fences: split into two atomics per value they were
-// slower (c128 N = 4096 19.4 -> 20.5 ms).
+// vectors, candidates).  NF (real types): agent-scope relaxed atomics,
+// coherent across the XCDs' L2s on their own, so that a record needs no L2
+// write-back / invalidate, only a wait for the block's stores before it
+// (N = 4096 f64 13.5 -> 12.7 ms, N = 8192 33.6 -> 30.9 ms, results
+// bit-identical).  Complex types keep plain accesses and agent-scope
+// fences: split into two atomics per value they were slower (c128 N = 4096
+// 19.4 -> 20.5 ms).
 template <bool NF, typename V>
 __device__ __forceinline__ void xb_st(V* p, V v) {
     if constexpr (NF) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -374,47 +396,45 @@ __device__ __forceinline__ V xb_ld(const V* p) {
 }
 template <typename T>
 constexpr bool panel_nf() { return !is_cplx<T>::value; }
-template <typename T>
-__device__ __forceinline__ void xb_st_part(BPPart<T>* p, const BPPart<T>& v) {
-    constexpr bool NF = panel_nf<T>();
-    if constexpr (NF) {
-        xb_st<NF>(&p->best, v.best); xb_st<NF>(&p->idx, v.idx); xb_st<NF>(&p->slot, v.slot); xb_st<NF>(&p->val, v.val);
-    } else {
-        *p = v;
-    }
-}
-template <typename T>
-__device__ __forceinline__ BPPart<T> xb_ld_part(const BPPart<T>* p) {
-    constexpr bool NF = panel_nf<T>();
-    if constexpr (NF) {
-        BPPart<T> v;
-        v.best = xb_ld<NF>(&p->best); v.idx = xb_ld<NF>(&p->idx); v.slot = xb_ld<NF>(&p->slot); v.val = xb_ld<NF>(&p->val);
-        return v;
-    } else {
-        return *p;
-    }
-}
 
-// Grid barrier: every block stores the epoch into its own flag (no shared
-// counter), and one wave polls all flags with one load per lane.
+// Wait for every block's record of epoch E and reduce them (one wave, one
+// block per lane): the winning block, or -1 when no block has a finite
+// candidate, with its logical index and slot.  A record that does not show
+// up within the spin limit sets err (the output becomes NaN); every record
+// word ever written decodes to indices in range, so nothing faults then.
 template <bool NF>
-__device__ __forceinline__ void bc_barrier(int* flags, int epoch, int* err) {
-    if constexpr (NF) __builtin_amdgcn_s_waitcnt(0);             // this thread's stores have landed
-    else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");      // this thread's stores are visible device-wide
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(&flags[blockIdx.x], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void bc_poll(const unsigned long long* rec_all, int e, int* err, int* s_w, int* s_p,
+                                        int* s_sp) {
     if (threadIdx.x < 64) {
         const int g = threadIdx.x;
+        const int G = gridDim.x;
+        const unsigned long long pb = bc_par(e);
+        const unsigned long long* rec = rec_all + ((int64_t)(e % 3) * BC_MAXG + (g < G ? g : 0)) * 2;
+        unsigned long long wa = BC_NONE_A, wb = BC_NONE_B;
         for (int spins = 0;; ++spins) {
-            int f = epoch;
-            if (g < (int)gridDim.x) f = __hip_atomic_load(&flags[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (__all(f >= epoch)) break;
+            if (g < G) {
+                wa = __hip_atomic_load(&rec[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                wb = __hip_atomic_load(&rec[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            const bool ok = g >= G || ((((wa ^ pb) | (wb ^ pb)) & BC_PAR) == 0);
+            if (__all(ok)) break;
             if (spins > (1 << 22)) {   // seconds: the grid is co-resident, so this is a failure
                 if (g == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
         }
+        if (g >= G) { wa = BC_NONE_A; wb = BC_NONE_B; }
+        const double m = __longlong_as_double((long long)(wa & ~BC_PAR));
+        const double best = (m == m) ? m : -1.0;
+        const int idx = (int)((wb >> 32) & 0x7fffffffull);
+        const int slot = (int)(unsigned)(wb & 0xffffffffull);
+        const double b = wave_max(best);                                       // inputs >= -1, no NaN
+        const int bi = wave_min((best == b && b >= 0.0) ? idx : 0x7fffffff);   // ties: lowest index
+        const unsigned long long hit = __ballot(b >= 0.0 && best == b && idx == bi);
+        const int w = hit ? (int)__builtin_ctzll(hit) : -1;
+        if (g == 0) *s_w = w;
+        if (w >= 0 && g == w) { *s_p = idx; *s_sp = slot; }
     }
     __syncthreads();
     if constexpr (!NF) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -430,13 +450,18 @@ __device__ __forceinline__ bool bc_better(double b, int i, double bb, int bi) {
     return b > bb || (b == bb && i < bi);
 }
 
-// Block argmax of (best, logical index) and publication of the winner:
-// partial -> part, its panel entries U_j, V_j (j < cnt) -> cand.
+// Block argmax of (best, logical index) and publication of the winner for
+// epoch e: its value and panel entries U_j, V_j (j < cnt) to the block's
+// candidate row, then, once they have landed, the record.
 template <typename T, int NB>
-__device__ __forceinline__ void bc_publish(double best, int bi, T val, int lg, int x, bool own, int cnt,
-                                           BPPart<T>* part, T* cand, const T* Ps, double* r_best, int* r_idx,
-                                           int* r_t) {
+__device__ __forceinline__ void bc_publish(double best, int bi, T val, int lg, int x, bool own, int cnt, int e,
+                                           const BPCoop<T>& co, const T* Ps, double* r_best, int* r_idx,
+                                           int* r_t, int* r_slot) {
+    constexpr bool NF = panel_nf<T>();
+    constexpr int CS = 2 * NB + 2;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int buf = e % 3;
+    T* cand = co.cand + ((int64_t)buf * BC_MAXG + blockIdx.x) * CS;
     const double b = wave_max(best);                                    // inputs >= -1, no NaN
     const int ix = wave_min((best == b && best >= 0.0) ? bi : 0x7fffffff);   // ties: lowest index
     if (lane == 0) { r_best[wv] = b; r_idx[wv] = ix; }
@@ -449,46 +474,54 @@ __device__ __forceinline__ void bc_publish(double best, int bi, T val, int lg, i
     const bool mine = found && own && best >= 0.0 && lg == gi;
     if (mine) {
         *r_t = t;
-        BPPart<T> pt;
-        pt.best = gb; pt.idx = gi; pt.slot = x; pt.val = val;
-        xb_st_part(part, pt);
-    } else if (!found && t == 0) {
-        BPPart<T> pt;
-        pt.best = -1.0; pt.idx = 0x7fffffff; pt.slot = -1; pt.val = zero<T>();
-        xb_st_part(part, pt);
+        *r_slot = x;
+        xb_st<NF>(&cand[2 * NB], val);
     }
     __syncthreads();
-    if (found && t < 2 * NB && (t >> 1) < cnt) xb_st<panel_nf<T>()>(&cand[t], Ps[2 * ((t >> 1) * BC_T + *r_t) + (t & 1)]);
+    if (found && t < 2 * NB && (t >> 1) < cnt) xb_st<NF>(&cand[t], Ps[2 * ((t >> 1) * BC_T + *r_t) + (t & 1)]);
+    if constexpr (NF) __builtin_amdgcn_s_waitcnt(0);             // this thread's stores have landed
+    else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");      // this thread's stores are visible device-wide
+    __syncthreads();
+    if (t == 0) {
+        const unsigned long long pb = bc_par(e);
+        unsigned long long wa = BC_NONE_A | pb, wb = BC_NONE_B | pb;
+        if (found) {
+            wa = (unsigned long long)__double_as_longlong(gb) | pb;   // gb >= +0: sign bit clear
+            wb = pb | ((unsigned long long)(unsigned)gi << 32) | (unsigned long long)(unsigned)*r_slot;
+        }
+        unsigned long long* rec = co.rec + ((int64_t)buf * BC_MAXG + blockIdx.x) * 2;
+        __hip_atomic_store(&rec[0], wa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&rec[1], wb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 template <typename T, int NB>
 __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int k0, int np, int epoch0) {
+    constexpr bool NF = panel_nf<T>();
+    constexpr int CS = 2 * NB + 2;
     extern __shared__ unsigned char smem_raw[];
     // Panel entries of the own slot, interleaved so that one 16-byte LDS read
     // returns a pair: U_j at Ps[2*(j*BC_T + t)], V_j at Ps[2*(j*BC_T + t) + 1].
     T* Ps = reinterpret_cast<T*>(smem_raw);
     __shared__ int Qpan[2 * NB + 2];           // slot of logical k0 + q
     // entries at one slot, interleaved (U_j at 2j, V_j at 2j + 1): the
-    // winner's (the new row k+1), and those at the pre-interchange slots of
-    // rows k+1 and k+2
-    // (one array, so that choosing one of them is an index, not a select
-    // between two loads): Bx[0] winner, Bx[1] pre-interchange k+1, Bx[2] k+2
-    __shared__ __align__(16) T Bx[3][2 * NB];
+    // winner's (the new row k+1, with its value at 2 NB), and those at the
+    // pre-interchange slots of rows k+1 and k+2 (one array, so that choosing
+    // one of them is an index, not a select between two loads): Bx[0]
+    // winner, Bx[1] pre-interchange k+1, Bx[2] k+2
+    __shared__ __align__(16) T Bx[3][CS];
     __shared__ double r_best[BC_WAVES];
-    __shared__ int r_idx[BC_WAVES], r_t;
+    __shared__ int r_idx[BC_WAVES], r_t, r_slot;
     __shared__ int s_w, s_p, s_sp;
-    __shared__ T s_piv;
 
     BPState<T>* st = a.st;
     if (st->dead) return;
     const int64_t n = a.n;
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int G = gridDim.x;
+    const int t = threadIdx.x;
     const int x = k0 + blockIdx.x * BC_T + t;   // own slot
     const bool own = x < n;
     T* PU = a.P;                                // U_j[x] at PU[j*n + x]
     T* PV = a.P + (int64_t)NB * n;              // V_j[x] at PV[j*n + x]
-    int nbar = 0;
 
     double lm = 0.0, ph_re = 0.0, ph_im = 0.0;   // pivot log / phase, kept by block 0 thread 0
     int le = 0;
@@ -499,119 +532,87 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
     int lg = x;                                 // logical index of the own slot
     for (int j = 0; j < NB; ++j) { Ps[2 * (j * BC_T + t)] = zero<T>(); Ps[2 * (j * BC_T + t) + 1] = zero<T>(); }
 
-    // first pivot row of the panel: row k0 of W (nothing pending)
+    // first pivot row of the panel: row k0 of W (nothing pending); epoch
+    // epoch0 + 1 + i carries the candidates of pair step i
     {
         double best = -1.0;
         T r = zero<T>();
         if (own && x > k0) {
             r = a.W[(int64_t)k0 * n + x];
             Ps[2 * t + 1] = r;
-            xb_st<panel_nf<T>()>(&PV[x], r);
+            xb_st<NF>(&PV[x], r);
             const double m = (double)mag2(r);
             if (m == m) best = m;
         }
-        bc_publish<T, NB>(best, lg, r, lg, x, own, 0, &co.part[blockIdx.x], co.cand + (int64_t)blockIdx.x * 2 * NB,
-                          Ps, r_best, r_idx, &r_t);
-        ++nbar;
-        bc_barrier<panel_nf<T>()>(co.flags, epoch0 + nbar, &st->err);
+        bc_publish<T, NB>(best, lg, r, lg, x, own, 0, epoch0 + 1, co, Ps, r_best, r_idx, &r_t, &r_slot);
     }
 
     for (int i = 0; i < np; ++i) {
         const int k = k0 + 2 * i;
+        const int e = epoch0 + 1 + i;
         const bool has_next = i + 1 < np;       // row k+2 is this panel's next pivot row
-        const int par = i & 1;
-        // ---- round 1: partials, every block's candidate entries, entries at the
-        //      pre-interchange slots of rows k+1 and k+2, and row k+2 of W
         const int sa = Qpan[k + 1 - k0];
         const int sb = has_next ? Qpan[k + 2 - k0] : -1;
-        // (every load below is unconditional from a valid address, masked after)
-        BPPart<T> pp = xb_ld_part(&co.part[par * BC_MAXG + (t < G ? t : 0)]);
-        if (t >= G) { pp.best = -1.0; pp.idx = 0x7fffffff; }
-        T cv[BC_CPW];
-        {
-            const bool lane_ok = lane < 2 * NB && (lane >> 1) < i;
-            const T* cb = co.cand + (int64_t)par * BC_MAXG * 2 * NB;
-#pragma unroll
-            for (int mm = 0; mm < BC_CPW; ++mm) {
-                const int g = wv + BC_WAVES * mm;
-                const bool ok = g < G && lane_ok;
-                const T v = xb_ld<panel_nf<T>()>(&cb[ok ? (int64_t)g * 2 * NB + lane : 0]);
-                cv[mm] = psel(ok, v, zero<T>());
-            }
-        }
-        const int xs = own ? x : k0;           // a valid slot for masked reads
+        bc_poll<NF>(co.rec, e, &st->err, &s_w, &s_p, &s_sp);
+        const int wb = s_w;
+        int p, sp;
+        if (wb >= 0) { p = s_p; sp = s_sp; }
+        else { p = k + 1; sp = sa; }                       // no finite candidate (NaN row)
+        const bool sw = p != k + 1;
+        // the interchange k+1 <-> p as seen from the own slot
+        if (x == sp) lg = k + 1;
+        else if (sw && x == sa) lg = p;
+        const int y1 = sp;
+        const int y2 = (p == k + 2) ? sa : sb;
+        const int y2s = has_next ? y2 : y1;
         const int sbs = sb >= 0 ? sb : sa;
+        const int xs = own ? x : k0;                       // a valid slot for masked reads
+
+        // ---- one round of loads (each unconditional from a valid address,
+        //      masked after): the winner's entries and value, the entries at
+        //      the pre-interchange slots of rows k+1 and k+2, and rows k+1
+        //      and k+2 of W at the own slots
+        if (t < 2 * NB + 1) {
+            const T* cb = co.cand + ((int64_t)(e % 3) * BC_MAXG + (wb >= 0 ? wb : 0)) * CS;
+            const bool ok = wb >= 0 && (t == 2 * NB || (t >> 1) < i);
+            const T v = xb_ld<NF>(&cb[t]);
+            Bx[0][t] = psel(ok, v, zero<T>());
+        }
         {
             const int tt = t & (2 * NB - 1);
             const int j = tt >> 1;
             const bool isv = tt & 1;
             const bool ok = j < i || (isv && j == i);
             const T* src = (isv ? PV : PU) + (int64_t)j * n;
-            const T v1 = xb_ld<panel_nf<T>()>(&src[sa]), v2 = xb_ld<panel_nf<T>()>(&src[sbs]);
+            const T v1 = xb_ld<NF>(&src[sa]), v2 = xb_ld<NF>(&src[sbs]);
             if (t < 2 * NB) {
                 Bx[1][t] = psel(ok, v1, zero<T>());
                 Bx[2][t] = psel(ok && sb >= 0, v2, zero<T>());
             }
         }
-        T w2 = wsl(a.W, n, sbs, xs);
-        if (!(has_next && own && lg >= k + 1)) w2 = zero<T>();
-        if (wv == 0) {
-            const double b = wave_max(pp.best);
-            const int bi = wave_min((pp.best == b && b >= 0.0) ? pp.idx : 0x7fffffff);
-            const unsigned long long hit = __ballot(b >= 0.0 && pp.best == b && pp.idx == bi);
-            if (t == 0) s_w = hit ? (int)__builtin_ctzll(hit) : -1;
-        }
-        __syncthreads();
-        const int wb = s_w;
-        if (wb >= 0 && t == wb) { s_p = pp.idx; s_sp = pp.slot; s_piv = pp.val; }
-        if (wb >= 0 && wv == (wb % BC_WAVES) && lane < 2 * NB) {
-            const int mw = wb / BC_WAVES;
-            T v = zero<T>();
-#pragma unroll
-            for (int mm = 0; mm < BC_CPW; ++mm)
-                if (mm == mw) v = cv[mm];
-            Bx[0][lane] = v;
-        }
-        __syncthreads();
-        int p, sp;
-        T piv;
-        {
-            const T pw = s_piv, pn = Bx[1][2 * i + 1];
-            if (wb >= 0) { p = s_p; sp = s_sp; }
-            else { p = k + 1; sp = sa; }                   // no finite candidate (NaN row)
-            piv = psel(wb >= 0, pw, pn);
-        }
-        if (is_zero(piv)) {                                // same decision in every block
-            if (keeper) st->dead = 1;
-            return;
-        }
-        const bool sw = p != k + 1;
-        if (keeper) accumulate(lm, le, ph_re, ph_im, piv, sw);
-        // ---- interchange k+1 <-> p (every read of Qpan in this step came
-        //      before the block barriers above)
-        if (sw && t == 0) {
-            Qpan[k + 1 - k0] = sp;
-            if (p - k0 < 2 * NB + 2) Qpan[p - k0] = sa;
-        }
-        if (x == sp) lg = k + 1;
-        else if (sw && x == sa) lg = p;
-        const int y1 = sp;
-        const int y2 = (p == k + 2) ? sa : sb;
-        const T* b1 = Bx[(wb >= 0) ? 0 : 1];
-        const T* b2 = Bx[(p == k + 2) ? 1 : 2];
-        if (k + 2 >= n) break;                             // last pair of the matrix
-
-        // ---- round 2: row k+1 of W at the own slots
         const bool act1 = own && lg >= k + 2;
-        const int y2s = has_next ? y2 : y1;
         T w1 = wsl(a.W, n, y1, xs);
         if (!act1) w1 = zero<T>();
         T w12 = wsl(a.W, n, y1, y2s);
         if (!has_next) w12 = zero<T>();
-        if (has_next && p == k + 2) {
-            w2 = wsl(a.W, n, y2, xs);
-            if (!(own && lg >= k + 3)) w2 = zero<T>();
+        T w2 = wsl(a.W, n, has_next ? y2 : y1, xs);
+        if (!(has_next && own && lg >= k + 3)) w2 = zero<T>();
+        __syncthreads();
+        const T piv = psel(wb >= 0, Bx[0][2 * NB], Bx[1][2 * i + 1]);
+        if (is_zero(piv)) {                                // same decision in every block
+            if (keeper) st->dead = 1;
+            return;
         }
+        if (keeper) accumulate(lm, le, ph_re, ph_im, piv, sw);
+        // the interchange in the shared map (every read of Qpan in this step
+        // came before the block barrier in bc_poll)
+        if (sw && t == 0) {
+            Qpan[k + 1 - k0] = sp;
+            if ((unsigned)(p - k0) < (unsigned)(2 * NB + 2)) Qpan[p - k0] = sa;
+        }
+        const T* b1 = Bx[(wb >= 0) ? 0 : 1];
+        const T* b2 = Bx[(p == k + 2) ? 1 : 2];
+        if (k + 2 >= n) break;                             // last pair of the matrix
 
         const T inv = recip(piv);
         // corrections of the pairs j < i, in chunks of eight (entries past i
@@ -645,7 +646,7 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         T u = zero<T>();
         if (act1) {
             u = (w1 + r1) * inv;
-            xb_st<panel_nf<T>()>(&PU[(int64_t)i * n + x], u);
+            xb_st<NF>(&PU[(int64_t)i * n + x], u);
         }
         Ps[2 * (i * BC_T + t)] = u;
         if (!has_next) break;
@@ -657,16 +658,12 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         T r = zero<T>();
         if (own && lg >= k + 3) {
             r = w2 + rl + u_kl * Ps[2 * (i * BC_T + t) + 1] - v_kl * u;
-            xb_st<panel_nf<T>()>(&PV[(int64_t)(i + 1) * n + x], r);
+            xb_st<NF>(&PV[(int64_t)(i + 1) * n + x], r);
             const double m = (double)mag2(r);
             if (m == m) best = m;
         }
         Ps[2 * ((i + 1) * BC_T + t) + 1] = r;
-        const int npar = (i + 1) & 1;
-        bc_publish<T, NB>(best, lg, r, lg, x, own, i + 1, &co.part[npar * BC_MAXG + blockIdx.x],
-                          co.cand + ((int64_t)npar * BC_MAXG + blockIdx.x) * 2 * NB, Ps, r_best, r_idx, &r_t);
-        ++nbar;
-        bc_barrier<panel_nf<T>()>(co.flags, epoch0 + nbar, &st->err);
+        bc_publish<T, NB>(best, lg, r, lg, x, own, i + 1, e + 1, co, Ps, r_best, r_idx, &r_t, &r_slot);
     }
 
     if (own) a.perm[lg] = x;   // logical -> slot, for the trailing update
@@ -1003,7 +1000,7 @@ static size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 
 template <typename T>
 struct BPLayout {
-    size_t w0, w1, p, r, perm, pv, pi, st, part, cand, bar, total;
+    size_t w0, w1, p, r, perm, pv, pi, st, rec, cand, total;
 };
 
 template <typename T>
@@ -1019,9 +1016,8 @@ static BPLayout<T> bp_layout(int64_t n, int nb) {
     L.pv = o; o = al(o + sizeof(double) * nblk);
     L.pi = o; o = al(o + sizeof(int) * nblk);
     L.st = o; o = al(o + sizeof(BPState<T>));
-    L.part = o; o = al(o + sizeof(BPPart<T>) * 2 * BC_MAXG);
-    L.cand = o; o = al(o + sizeof(T) * 2 * BC_MAXG * 2 * nb);
-    L.bar = o; o = al(o + sizeof(int) * BC_MAXG);
+    L.rec = o; o = al(o + sizeof(unsigned long long) * 3 * BC_MAXG * 2);
+    L.cand = o; o = al(o + sizeof(T) * 3 * BC_MAXG * (2 * nb + 2));
     L.total = o;
     return L;
 }
@@ -1114,14 +1110,13 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
     a.n = n;
     a.nb = nb;
     BPCoop<T> co{};
-    co.part = (BPPart<T>*)(base + L.part);
+    co.rec = (unsigned long long*)(base + L.rec);
     co.cand = (T*)(base + L.cand);
-    co.flags = (int*)(base + L.bar);
     const bool coop = coop_enabled() && nb == (std::is_same<T, c128>::value ? 16 : 32);
     hipError_t e = hipMemcpyAsync(W[0], A, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return (int)e;
     a.W = W[0];
-    hipLaunchKernelGGL(bp_init<T>, dim3(64), dim3(256), 0, s, a, co.flags);
+    hipLaunchKernelGGL(bp_init<T>, dim3(64), dim3(256), 0, s, a, co.rec);
     int cur = 0;
     int epoch = 0;   // grid barriers passed so far (cooperative panels)
     auto blocks = [&](int64_t c_lo) { return (unsigned)((n - c_lo + BP_THREADS - 1) / BP_THREADS); };
@@ -1663,9 +1658,8 @@ int blocked_grad(const T* A, int64_t n, void* ws, double* log_out, T* phase_out,
     a.n = n;
     a.nb = NB;
     BPCoop<T> co{};
-    co.part = (BPPart<T>*)(base + L.bp.part);
+    co.rec = (unsigned long long*)(base + L.bp.rec);
     co.cand = (T*)(base + L.bp.cand);
-    co.flags = (int*)(base + L.bp.bar);
     T* S = (T*)(base + L.s);
     T* Z = (T*)(base + L.z);
     T* Ai = (T*)(base + L.ai);
@@ -1674,7 +1668,7 @@ int blocked_grad(const T* A, int64_t n, void* ws, double* log_out, T* phase_out,
     hipError_t e = hipMemcpyAsync(W[0], A, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return (int)e;
     a.W = W[0];
-    hipLaunchKernelGGL(bp_init<T>, dim3(64), dim3(256), 0, s, a, co.flags);
+    hipLaunchKernelGGL(bp_init<T>, dim3(64), dim3(256), 0, s, a, co.rec);
     hipLaunchKernelGGL(bp_reset_perm<T>, dim3(64), dim3(256), 0, s, gp[0], (int64_t)0, n);
     const int ntile = (int)((n + BP_TILE - 1) / BP_TILE);
     const int tiles = ntile * (ntile + 1) / 2;
