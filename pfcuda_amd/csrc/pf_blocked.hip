@@ -344,11 +344,11 @@ __global__ void __launch_bounds__(BP_THREADS) bp_row(BPArgs<T> a, int k, int i, 
 // are then by slot.
 //
 // After computing the next pivot row, every block publishes its best
-// candidate (|value|, logical index, slot, value) together with that slot's
-// panel entries U_j, V_j; after the barrier every block reduces the
-// partials itself, so the pivot row's correction coefficients arrive in the
-// same round trip as the argmax.  Partials and candidate entries are
-// double-buffered by pair-step parity.
+// candidate (|value|, logical index, slot) as a record that is also its
+// barrier flag: every block polls all records, reduces them itself, and then
+// issues one round of loads for everything the pair step needs (the pivot's
+// panel entries and value, the entries at the pre-interchange slots of rows
+// k+1 and k+2, and those two rows of W at its own slots).
 // ---------------------------------------------------------------------------
 
 // U_j, V_j of one slot, adjacent in the panel's LDS store
@@ -367,17 +367,25 @@ struct alignas(2 * sizeof(T)) BCPair {
 // publishing E has seen every record E - 1, each published after its
 // author's reads of E - 2 and earlier.  So a reader polling for E sees either
 // E - 3 (keeps polling) or E, word by word, and once both words show E, the
-// block's other exchange stores (its candidate's panel entries and value,
-// its panel vectors) have landed: they precede the record's stores.
+// block's other exchange stores have landed: they precede the record's
+// stores.
+//
+// The readers need the winner's entries (U_j, V_j for j < i, and its value
+// V_i).  Real types: the winning block copies them into its candidate row
+// (buffer E % 3, the same reuse argument), one contiguous read for the
+// others.  Complex types: the readers gather them from the panel vectors at
+// the winner's slot, as they gather those of rows k+1 and k+2.  Measured
+// (ms, gather / row): f64 N = 8192 30.3 / 25.3, N = 4096 12.4 / 9.8;
+// c128 N = 4096 15.4 / 16.8.
 
 template <typename T>
 struct BPCoop {
     unsigned long long* rec;   // [3][BC_MAXG][2]: records A, B by epoch % 3
-    T* cand;                   // [3][BC_MAXG][2 nb + 2]: the candidate's U_j at 2j, V_j at 2j + 1, its value at 2 nb
+    T* cand;                   // [3][BC_MAXG][2 nb]: a block's candidate entries (real types), by epoch % 3
 };
 
 // Accesses to the data the panel's blocks exchange within a launch (panel
-// vectors, candidates).  NF (real types): agent-scope relaxed atomics,
+// vectors).  NF (real types): agent-scope relaxed atomics,
 // coherent across the XCDs' L2s on their own, so that a record needs no L2
 // write-back / invalidate, only a wait for the block's stores before it
 // (N = 4096 f64 13.5 -> 12.7 ms, N = 8192 33.6 -> 30.9 ms, results
@@ -396,6 +404,8 @@ __device__ __forceinline__ V xb_ld(const V* p) {
 }
 template <typename T>
 constexpr bool panel_nf() { return !is_cplx<T>::value; }
+template <typename T>
+constexpr bool panel_cand() { return !is_cplx<T>::value; }   // the winner's entries through a candidate row
 
 // Wait for every block's record of epoch E and reduce them (one wave, one
 // block per lane): the winning block, or -1 when no block has a finite
@@ -451,17 +461,16 @@ __device__ __forceinline__ bool bc_better(double b, int i, double bb, int bi) {
 }
 
 // Block argmax of (best, logical index) and publication of the winner for
-// epoch e: its value and panel entries U_j, V_j (j < cnt) to the block's
-// candidate row, then, once they have landed, the record.
+// epoch e: with candidate rows, its entries U_j, V_j (j < cnt) and its value
+// (V_cnt) first; then, once this block's exchange stores have landed, the
+// record.
 template <typename T, int NB>
 __device__ __forceinline__ void bc_publish(double best, int bi, T val, int lg, int x, bool own, int cnt, int e,
                                            const BPCoop<T>& co, const T* Ps, double* r_best, int* r_idx,
                                            int* r_t, int* r_slot) {
     constexpr bool NF = panel_nf<T>();
-    constexpr int CS = 2 * NB + 2;
+    constexpr bool CAND = panel_cand<T>();
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int buf = e % 3;
-    T* cand = co.cand + ((int64_t)buf * BC_MAXG + blockIdx.x) * CS;
     const double b = wave_max(best);                                    // inputs >= -1, no NaN
     const int ix = wave_min((best == b && best >= 0.0) ? bi : 0x7fffffff);   // ties: lowest index
     if (lane == 0) { r_best[wv] = b; r_idx[wv] = ix; }
@@ -471,14 +480,18 @@ __device__ __forceinline__ void bc_publish(double best, int bi, T val, int lg, i
     for (int u = 1; u < BC_WAVES; ++u)
         if (bc_better(r_best[u], r_idx[u], gb, gi)) { gb = r_best[u]; gi = r_idx[u]; }
     const bool found = gb >= 0.0;
-    const bool mine = found && own && best >= 0.0 && lg == gi;
-    if (mine) {
-        *r_t = t;
+    T* cand = co.cand + ((int64_t)(e % 3) * BC_MAXG + blockIdx.x) * 2 * NB;
+    if (found && own && best >= 0.0 && lg == gi) {
         *r_slot = x;
-        xb_st<NF>(&cand[2 * NB], val);
+        if constexpr (CAND) {
+            *r_t = t;
+            xb_st<NF>(&cand[2 * cnt + 1], val);
+        }
     }
-    __syncthreads();
-    if (found && t < 2 * NB && (t >> 1) < cnt) xb_st<NF>(&cand[t], Ps[2 * ((t >> 1) * BC_T + *r_t) + (t & 1)]);
+    if constexpr (CAND) {
+        __syncthreads();
+        if (found && t < 2 * cnt) xb_st<NF>(&cand[t], Ps[2 * ((t >> 1) * BC_T + *r_t) + (t & 1)]);
+    }
     if constexpr (NF) __builtin_amdgcn_s_waitcnt(0);             // this thread's stores have landed
     else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");      // this thread's stores are visible device-wide
     __syncthreads();
@@ -489,7 +502,7 @@ __device__ __forceinline__ void bc_publish(double best, int bi, T val, int lg, i
             wa = (unsigned long long)__double_as_longlong(gb) | pb;   // gb >= +0: sign bit clear
             wb = pb | ((unsigned long long)(unsigned)gi << 32) | (unsigned long long)(unsigned)*r_slot;
         }
-        unsigned long long* rec = co.rec + ((int64_t)buf * BC_MAXG + blockIdx.x) * 2;
+        unsigned long long* rec = co.rec + ((int64_t)(e % 3) * BC_MAXG + blockIdx.x) * 2;
         __hip_atomic_store(&rec[0], wa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&rec[1], wb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -498,18 +511,17 @@ __device__ __forceinline__ void bc_publish(double best, int bi, T val, int lg, i
 template <typename T, int NB>
 __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int k0, int np, int epoch0) {
     constexpr bool NF = panel_nf<T>();
-    constexpr int CS = 2 * NB + 2;
     extern __shared__ unsigned char smem_raw[];
     // Panel entries of the own slot, interleaved so that one 16-byte LDS read
     // returns a pair: U_j at Ps[2*(j*BC_T + t)], V_j at Ps[2*(j*BC_T + t) + 1].
     T* Ps = reinterpret_cast<T*>(smem_raw);
     __shared__ int Qpan[2 * NB + 2];           // slot of logical k0 + q
     // entries at one slot, interleaved (U_j at 2j, V_j at 2j + 1): the
-    // winner's (the new row k+1, with its value at 2 NB), and those at the
+    // winner's (the new row k+1, its value V_i at 2i + 1), and those at the
     // pre-interchange slots of rows k+1 and k+2 (one array, so that choosing
     // one of them is an index, not a select between two loads): Bx[0]
     // winner, Bx[1] pre-interchange k+1, Bx[2] k+2
-    __shared__ __align__(16) T Bx[3][CS];
+    __shared__ __align__(16) T Bx[3][2 * NB];
     __shared__ double r_best[BC_WAVES];
     __shared__ int r_idx[BC_WAVES], r_t, r_slot;
     __shared__ int s_w, s_p, s_sp;
@@ -572,23 +584,18 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         //      masked after): the winner's entries and value, the entries at
         //      the pre-interchange slots of rows k+1 and k+2, and rows k+1
         //      and k+2 of W at the own slots
-        if (t < 2 * NB + 1) {
-            const T* cb = co.cand + ((int64_t)(e % 3) * BC_MAXG + (wb >= 0 ? wb : 0)) * CS;
-            const bool ok = wb >= 0 && (t == 2 * NB || (t >> 1) < i);
-            const T v = xb_ld<NF>(&cb[t]);
-            Bx[0][t] = psel(ok, v, zero<T>());
-        }
-        {
-            const int tt = t & (2 * NB - 1);
-            const int j = tt >> 1;
-            const bool isv = tt & 1;
+        if (t < 2 * NB) {                                  // the lanes that keep them
+            const int j = t >> 1;
+            const bool isv = t & 1;
             const bool ok = j < i || (isv && j == i);
             const T* src = (isv ? PV : PU) + (int64_t)j * n;
-            const T v1 = xb_ld<NF>(&src[sa]), v2 = xb_ld<NF>(&src[sbs]);
-            if (t < 2 * NB) {
-                Bx[1][t] = psel(ok, v1, zero<T>());
-                Bx[2][t] = psel(ok && sb >= 0, v2, zero<T>());
-            }
+            const T* w0 = panel_cand<T>()
+                ? co.cand + ((int64_t)(e % 3) * BC_MAXG + (wb >= 0 ? wb : 0)) * 2 * NB + t
+                : &src[sp];
+            const T v0 = xb_ld<NF>(w0), v1 = xb_ld<NF>(&src[sa]), v2 = xb_ld<NF>(&src[sbs]);
+            Bx[0][t] = psel(ok, v0, zero<T>());
+            Bx[1][t] = psel(ok, v1, zero<T>());
+            Bx[2][t] = psel(ok && sb >= 0, v2, zero<T>());
         }
         const bool act1 = own && lg >= k + 2;
         T w1 = wsl(a.W, n, y1, xs);
@@ -598,7 +605,8 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         T w2 = wsl(a.W, n, has_next ? y2 : y1, xs);
         if (!(has_next && own && lg >= k + 3)) w2 = zero<T>();
         __syncthreads();
-        const T piv = psel(wb >= 0, Bx[0][2 * NB], Bx[1][2 * i + 1]);
+        const T* b1 = Bx[(wb >= 0) ? 0 : 1];               // the pivot's entries (row k+1 at the start)
+        const T piv = b1[2 * i + 1];                       // V_i there
         if (is_zero(piv)) {                                // same decision in every block
             if (keeper) st->dead = 1;
             return;
@@ -610,7 +618,6 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
             Qpan[k + 1 - k0] = sp;
             if ((unsigned)(p - k0) < (unsigned)(2 * NB + 2)) Qpan[p - k0] = sa;
         }
-        const T* b1 = Bx[(wb >= 0) ? 0 : 1];
         const T* b2 = Bx[(p == k + 2) ? 1 : 2];
         if (k + 2 >= n) break;                             // last pair of the matrix
 
@@ -1017,7 +1024,7 @@ static BPLayout<T> bp_layout(int64_t n, int nb) {
     L.pi = o; o = al(o + sizeof(int) * nblk);
     L.st = o; o = al(o + sizeof(BPState<T>));
     L.rec = o; o = al(o + sizeof(unsigned long long) * 3 * BC_MAXG * 2);
-    L.cand = o; o = al(o + sizeof(T) * 3 * BC_MAXG * (2 * nb + 2));
+    L.cand = o; o = al(o + sizeof(T) * 3 * BC_MAXG * 2 * nb);
     L.total = o;
     return L;
 }
