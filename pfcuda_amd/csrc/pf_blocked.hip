@@ -376,7 +376,10 @@ struct alignas(2 * sizeof(T)) BCPair {
 // others.  Complex types: the readers gather them from the panel vectors at
 // the winner's slot, as they gather those of rows k+1 and k+2.  Measured
 // (ms, gather / row): f64 N = 8192 30.3 / 25.3, N = 4096 12.4 / 9.8;
-// c128 N = 4096 15.4 / 16.8.
+// c128 N = 4096 15.4 / 16.8.  A slot-major copy of the panel vectors, which
+// makes every one of these reads contiguous but every thread's two new
+// entries a store to its own line, was slower for f64 too (30.9 ms at
+// N = 8192; c128 15.9 ms).
 
 template <typename T>
 struct BPCoop {
