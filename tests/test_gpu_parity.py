@@ -151,6 +151,38 @@ def test_ragged_vs_oracle(pf, cuda, dtype):
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
+def test_ragged_above_256_and_beyond_max_n(pf, cuda, dtype):
+    # Sizes past the batched kernels go to the global-slab bucket of the
+    # same call; an odd one gives (-inf, 0).  A size past the caller's max_n
+    # (which sizes the workspace) gives NaN and touches nothing, and the
+    # other matrices of the call are unaffected.
+    rng = np.random.default_rng(256)
+    cplx = dtype.is_complex
+    sizes = [258, 300, 400, 301, 16, 130, 256, 64]
+    mats = [oracle.skew_gaussian(rng, n, complex_=cplx) for n in sizes]
+    data = np.concatenate([m.reshape(-1) for m in mats])
+    offs = np.cumsum([0] + [n * n for n in sizes[:-1]])
+    gl, gp = pf.slog_pfaffian_ragged(torch.from_numpy(data).to(cuda), offs, sizes)
+    gl, gp = gl.cpu().numpy(), gp.cpu().numpy()
+    wl, wp = oracle.slog_ragged(data, offs, sizes, threads=8)
+    for i, n in enumerate(sizes):
+        if n & 1:
+            assert gl[i] == -math.inf and gp[i] == 0, (i, n)
+        else:
+            assert abs(gl[i] - wl[i]) < 1e-9 and abs(gp[i] - wp[i]) < 1e-9, (i, n, gl[i], wl[i])
+    sizes2 = [100, 300, 320]
+    mats2 = [oracle.skew_gaussian(rng, n, complex_=cplx) for n in sizes2]
+    data2 = np.concatenate([m.reshape(-1) for m in mats2])
+    offs2 = np.cumsum([0] + [n * n for n in sizes2[:-1]])
+    gl2, gp2 = pf.slog_pfaffian_ragged(torch.from_numpy(data2).to(cuda), offs2, sizes2, max_n=300)
+    gl2, gp2 = gl2.cpu().numpy(), gp2.cpu().numpy()
+    for i in (0, 1):
+        l0, p0 = oracle.slog(mats2[i])
+        assert abs(gl2[i] - l0) < 1e-9 and abs(gp2[i] - p0) < 1e-9, (i, gl2[i], l0)
+    assert math.isnan(gl2[2]) and np.isnan(gp2[2])
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
 @pytest.mark.parametrize("n", [300, 512])
 def test_unblocked_large_vs_oracle(pf, cuda, dtype, n):
     rng = np.random.default_rng(n)
