@@ -41,7 +41,7 @@ namespace pfh {
 constexpr int BP_THREADS = 128;    // columns per block in the row kernels
 constexpr int BP_TILE = 64;        // trailing-update tile edge
 constexpr int BC_T = 256;          // cooperative panel: threads per block = slots per block
-constexpr int BC_MAXG = 64;        // cooperative panel: blocks at most (one wave reduces the partials)
+constexpr int BC_MAXG = 64;        // cooperative panel: blocks at most (one wave polls and reduces their records)
 constexpr int BC_WAVES = BC_T / 64;
 // candidate records (described with bc_poll): the top bit carries the
 // epoch's parity; "no candidate" is NaN bits in A and index 2^31 - 1, slot -1
