@@ -41,6 +41,29 @@ namespace pfh {
 constexpr int BP_THREADS = 128;    // columns per block in the row kernels
 constexpr int BP_TILE = 64;        // trailing-update tile edge
 constexpr int BC_T = 256;          // cooperative panel: threads per block = slots per block
+
+// Upper 64 x 64 tile (ti <= tj) of block b, in an order that keeps each tile
+// column on one XCD: workgroups go to the 8 XCDs round-robin, and XCD
+// x = b % 8 takes the columns tj = x, x + 8, ..., each column's tiles
+// ti = 0..tj in turn, so a column's operand (the same for all its tiles)
+// is read into that XCD's L2 once.  False for the blocks past the last tile
+// of their XCD (the grid is 8 times the largest XCD share).
+__device__ __forceinline__ bool xcd_tile(int b, int ntile, int& ti, int& tj) {
+    int rem = b >> 3;
+    tj = b & 7;
+    while (tj < ntile && rem > tj) { rem -= tj + 1; tj += 8; }
+    ti = rem;
+    return tj < ntile;
+}
+static int xcd_tile_blocks(int ntile) {
+    int most = 0;
+    for (int x = 0; x < 8; ++x) {
+        int c = 0;
+        for (int tj = x; tj < ntile; tj += 8) c += tj + 1;
+        most = c > most ? c : most;
+    }
+    return 8 * most;
+}
 constexpr int BC_MAXG = 64;        // cooperative panel: blocks at most (one wave polls and reduces their records)
 constexpr int BC_WAVES = BC_T / 64;
 // candidate records (described with bc_poll): the top bit carries the
@@ -721,10 +744,8 @@ __global__ void __launch_bounds__(256) bp_trailing(const T* __restrict__ Wo, T* 
     T* Vr = Ur + BP_TILE * ld;
     T* Uc = Vr + BP_TILE * ld;
     T* Vc = Uc + BP_TILE * ld;
-    // upper-triangular tile index -> (ti, tj), tj >= ti
-    int t = blockIdx.x, ti = 0;
-    while (t >= ntile - ti) { t -= ntile - ti; ++ti; }
-    const int tj = ti + t;
+    int ti, tj;
+    if (!xcd_tile(blockIdx.x, ntile, ti, tj)) return;
     const int r0 = lo + ti * BP_TILE, c0 = lo + tj * BP_TILE;
     for (int e = threadIdx.x; e < BP_TILE * np; e += 256) {
         const int q = e % BP_TILE, j = e / BP_TILE;   // consecutive threads: consecutive rows (coalesced)
@@ -797,9 +818,8 @@ __global__ void __launch_bounds__(256) bp_trailing_mfma(const double* __restrict
                                                         const double* __restrict__ P, const int* __restrict__ perm,
                                                         int64_t n, int lo, int np, int ntile, int pslot) {
     constexpr int K = 2 * NB;
-    int tb = blockIdx.x, ti = 0;
-    while (tb >= ntile - ti) { tb -= ntile - ti; ++ti; }
-    const int tj = ti + tb;
+    int ti, tj;
+    if (!xcd_tile(blockIdx.x, ntile, ti, tj)) return;
     const int r0 = lo + ti * BP_TILE + 16 * (threadIdx.x >> 6), c0 = lo + tj * BP_TILE;
     const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
     const int n1 = (int)n - 1;
@@ -864,9 +884,8 @@ __global__ void __launch_bounds__(256) bp_trailing_mfma_c(const c128* __restrict
                                                           const c128* __restrict__ P, const int* __restrict__ perm,
                                                           int64_t n, int lo, int np, int ntile, int pslot) {
     constexpr int K = 2 * NB;
-    int tb = blockIdx.x, ti = 0;
-    while (tb >= ntile - ti) { tb -= ntile - ti; ++ti; }
-    const int tj = ti + tb;
+    int ti, tj;
+    if (!xcd_tile(blockIdx.x, ntile, ti, tj)) return;
     const int r0 = lo + ti * BP_TILE + 16 * (threadIdx.x >> 6), c0 = lo + tj * BP_TILE;
     const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
     const int n1 = (int)n - 1;
@@ -949,9 +968,8 @@ __global__ void __launch_bounds__(256) bp_trailing_staged(const T* __restrict__ 
                                                           int64_t n, int lo, int np, int ntile, int pslot) {
     constexpr int K = 2 * NB;
     __shared__ T Bs[K][BP_TILE];
-    int tb = blockIdx.x, ti = 0;
-    while (tb >= ntile - ti) { tb -= ntile - ti; ++ti; }
-    const int tj = ti + tb;
+    int ti, tj;
+    if (!xcd_tile(blockIdx.x, ntile, ti, tj)) return;
     const int r0 = lo + ti * BP_TILE + 16 * (threadIdx.x >> 6), c0 = lo + tj * BP_TILE;
     const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
     const int col = threadIdx.x & 63, kq = threadIdx.x >> 6;   // staging: column, first K row
@@ -1185,7 +1203,7 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
         const int64_t lo = k0 + 2 * np;
         if (lo + 1 < n) {
             const int ntile = (int)((n - lo + BP_TILE - 1) / BP_TILE);
-            const int tiles = ntile * (ntile + 1) / 2;
+            const int tiles = xcd_tile_blocks(ntile);
             const size_t smem = sizeof(T) * 4 * BP_TILE * (np + 1);
             if (smem > 65536) {
                 e = hipFuncSetAttribute((const void*)bp_trailing<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1362,12 +1380,9 @@ __global__ void __launch_bounds__(256) gj_rows(const T* __restrict__ W, const in
     }
 }
 
-// upper 64 x 64 tile index -> (ti, tj), tj >= ti
-__device__ __forceinline__ void gj_tile(int ntile, int& ti, int& tj) {
-    int tb = blockIdx.x;
-    ti = 0;
-    while (tb >= ntile - ti) { tb -= ntile - ti; ++ti; }
-    tj = ti + tb;
+// upper 64 x 64 tile (ti, tj), tj >= ti, of this block (xcd_tile's order)
+__device__ __forceinline__ bool gj_tile(int ntile, int& ti, int& tj) {
+    return xcd_tile(blockIdx.x, ntile, ti, tj);
 }
 
 // any element type: thread (tx, ty) owns rows r0+ty+16a, cols c0+tx+16b
@@ -1378,7 +1393,7 @@ __global__ void __launch_bounds__(256) gj_trailing(const T* __restrict__ Wo, T* 
                                                    const BPState<T>* st) {
     if (st->dead) return;
     int ti, tj;
-    gj_tile(ntile, ti, tj);
+    if (!gj_tile(ntile, ti, tj)) return;
     const int r0 = ti * BP_TILE, c0 = tj * BP_TILE;
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4, n1 = (int)n - 1;
     int rr[4], cc[4];
@@ -1425,7 +1440,7 @@ __global__ void __launch_bounds__(256) gj_trailing_mfma(const double* __restrict
                                                         int ntile, const BPState<double>* st) {
     if (st->dead) return;
     int ti, tj;
-    gj_tile(ntile, ti, tj);
+    if (!gj_tile(ntile, ti, tj)) return;
     const int r0 = ti * BP_TILE + 16 * (threadIdx.x >> 6), c0 = tj * BP_TILE;
     const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
     const int n1 = (int)n - 1;
@@ -1491,7 +1506,7 @@ __global__ void __launch_bounds__(256) gj_trailing_mfma_c(const c128* __restrict
                                                           int ntile, const BPState<c128>* st) {
     if (st->dead) return;
     int ti, tj;
-    gj_tile(ntile, ti, tj);
+    if (!gj_tile(ntile, ti, tj)) return;
     const int r0 = ti * BP_TILE + 16 * (threadIdx.x >> 6), c0 = tj * BP_TILE;
     const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
     const int n1 = (int)n - 1;
@@ -1551,7 +1566,7 @@ __global__ void __launch_bounds__(256) gj_trailing_staged(const T* __restrict__ 
     __shared__ T Bs[M][BP_TILE];
     if (st->dead) return;
     int ti, tj;
-    gj_tile(ntile, ti, tj);
+    if (!gj_tile(ntile, ti, tj)) return;
     const int r0 = ti * BP_TILE + 16 * (threadIdx.x >> 6), c0 = tj * BP_TILE;
     const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
     const int col = threadIdx.x & 63, kq = threadIdx.x >> 6;
@@ -1707,7 +1722,7 @@ int blocked_grad(const T* A, int64_t n, void* ws, double* log_out, T* phase_out,
     hipLaunchKernelGGL(bp_init<T>, dim3(64), dim3(256), 0, s, a, co.rec);
     hipLaunchKernelGGL(bp_reset_perm<T>, dim3(64), dim3(256), 0, s, gp[0], (int64_t)0, n);
     const int ntile = (int)((n + BP_TILE - 1) / BP_TILE);
-    const int tiles = ntile * (ntile + 1) / 2;
+    const int tiles = xcd_tile_blocks(ntile);
     int cur = 0, gc = 0, epoch = 0;
     for (int64_t k0 = 0; k0 + 1 < n; k0 += 2 * NB) {
         const int np = (int)((n - k0) / 2 < NB ? (n - k0) / 2 : NB);
