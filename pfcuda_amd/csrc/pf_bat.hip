@@ -343,15 +343,7 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
 
 namespace {
 
-int bat_num_cu() {
-    static int v = 0;
-    if (v == 0) {
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
-    }
-    return v;
-}
+int bat_num_cu() { return device_cu_count(); }
 
 template <typename T, int NP, int NB>
 constexpr size_t bat_lds_bytes() { return sizeof(T) * 2 * NB * NP; }
@@ -360,8 +352,8 @@ constexpr size_t bat_lds_bytes() { return sizeof(T) * 2 * NB * NP; }
 // which the occupancy query needs
 template <typename T, int NP, int NB>
 int bat_grid() {
-    static int g = 0;
-    if (g == 0) {
+    static DevCache resident;
+    return resident.get([] {
         (void)hipFuncSetAttribute((const void*)pf_bat_kernel<T, NP, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)bat_lds_bytes<T, NP, NB>());
         int occ = 0;
@@ -369,9 +361,8 @@ int bat_grid() {
                                                          bat_lds_bytes<T, NP, NB>()) != hipSuccess ||
             occ <= 0)
             occ = 1;
-        g = occ * bat_num_cu();
-    }
-    return g;
+        return occ * bat_num_cu();
+    });
 }
 
 // doubles of one workgroup's slab pair
@@ -383,12 +374,13 @@ int bat_launch(const BatchDesc& d, hipStream_t s) {
     if (d.count <= 0) return 0;
     auto kern = pf_bat_kernel<T, NP, NB>;
     constexpr size_t bytes = bat_lds_bytes<T, NP, NB>();
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-        if (e != hipSuccess) return (int)e;
-        attr = true;
-    }
+    static DevCache attr;   // the LDS limit is set on the device
+    hipError_t e = hipSuccess;
+    if (!attr.get([&] {
+            e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+            return e == hipSuccess ? 1 : 0;
+        }))
+        return (int)e;
     int64_t grid = bat_grid<T, NP, NB>();
     constexpr int64_t per = bat_per<T, NP>();   // one slab pair per workgroup
     if (!d.bslab || d.bslab_elems < per) return (int)hipErrorInvalidValue;

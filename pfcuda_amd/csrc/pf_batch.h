@@ -4,7 +4,49 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+
 namespace pfh {
+
+// Per-device caches of facts the launchers look up on every call (the CU
+// count, grids sized from occupancy, dynamic-LDS limits already set): one
+// atomic slot per device, filled on first use with that device current, so
+// that host threads calling at once and processes that switch devices see
+// the current device's value.  A slot holds 0 until known; get() stores
+// only the nonzero results of f.
+constexpr int kMaxDevices = 64;
+
+inline int current_device() {
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess || d < 0) d = 0;
+    return d < kMaxDevices ? d : kMaxDevices - 1;
+}
+
+struct DevCache {
+    std::atomic<int> v[kMaxDevices];
+    template <typename F>
+    int get(F f) {
+        std::atomic<int>& slot = v[current_device()];
+        int x = slot.load(std::memory_order_acquire);
+        if (x == 0) {
+            x = f();
+            if (x != 0) slot.store(x, std::memory_order_release);
+        }
+        return x;
+    }
+};
+
+// Multiprocessor (CU) count of the current device (256 when the query fails).
+inline int device_cu_count() {
+    static DevCache cache;
+    return cache.get([] {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+            v = 256;
+        return v;
+    });
+}
 
 // One batch of matrices for the workgroup kernels.  Matrix j of the launch
 // is order[j] when `order` is set (a size bucket), else j.  Ragged batches

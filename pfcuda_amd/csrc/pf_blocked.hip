@@ -1087,30 +1087,30 @@ int blocked_panel_width(int dtype) {
 // CU, fit at once), unless PFH_BLOCKED_COOP=0, and while the panel's
 // columns fit BC_MAXG blocks of BC_T slots.
 static bool coop_enabled() {
-    static int v = -1;
-    if (v < 0) {
+    static DevCache cache;   // 1 enabled, 2 not
+    return cache.get([] {
         const char* e = std::getenv("PFH_BLOCKED_COOP");
         int dev = 0, ok = 0, cus = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&ok, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             ok = 0;
-        v = (ok && cus >= BC_MAXG && !(e && e[0] == '0')) ? 1 : 0;
-    }
-    return v == 1;
+        return (ok && cus >= BC_MAXG && !(e && e[0] == '0')) ? 1 : 2;
+    }) == 1;
 }
 
 template <typename T, int NB>
 static hipError_t launch_panel(const BPArgs<T>& a, const BPCoop<T>& co, int k0, int np, int epoch0, int G,
                                hipStream_t s) {
     const size_t smem = sizeof(T) * 2 * NB * BC_T;
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void*)bp_panel<T, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)smem);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
+    static DevCache attr;   // the LDS limit is set on the device
+    hipError_t err = hipSuccess;
+    if (!attr.get([&] {
+            err = hipFuncSetAttribute((const void*)bp_panel<T, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)smem);
+            return err == hipSuccess ? 1 : 0;
+        }))
+        return err;
     // Plain launch by default: G <= BC_MAXG blocks at one per CU (their LDS)
     // on a device with more CUs than that are resident together once the
     // stream's earlier work has drained.  A block that cannot become

@@ -178,12 +178,7 @@ struct RaggedLayout {
     int64_t slab_elems;
 };
 
-int glb_grid() {
-    int dev = 0, v = 0;
-    (void)hipGetDevice(&dev);
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
-    return 2 * v;
-}
+int glb_grid() { return 2 * device_cu_count(); }
 
 RaggedLayout ragged_layout(int dt, int64_t count, int32_t max_n) {
     RaggedLayout L{};

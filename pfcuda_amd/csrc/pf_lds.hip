@@ -397,15 +397,7 @@ pf_lds_kernel(BatchDesc d) {
 
 namespace {
 
-int lds_num_cu() {
-    static int v = 0;
-    if (v == 0) {
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
-    }
-    return v;
-}
+int lds_num_cu() { return device_cu_count(); }
 
 template <int NP>
 constexpr size_t lds_bytes() {
@@ -417,15 +409,16 @@ int lds_launch(const BatchDesc& d, hipStream_t s) {
     if (d.count <= 0) return 0;
     auto kern = pf_lds_kernel<NP, W, MODE>;
     constexpr size_t bytes = lds_bytes<NP>();
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-        if (e != hipSuccess) return (int)e;
-        attr = true;
-    }
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, W * 64, bytes) != hipSuccess || occ <= 0) occ = 1;
-    int64_t grid = (int64_t)occ * lds_num_cu();
+    static DevCache resident;   // resident workgroups on the device, once its LDS limit is set
+    hipError_t e = hipSuccess;
+    int64_t grid = resident.get([&] {
+        e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return 0;
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, W * 64, bytes) != hipSuccess || occ <= 0) occ = 1;
+        return occ * lds_num_cu();
+    });
+    if (grid == 0) return (int)e;
     if (NP > 192) {
         // one spill slab per workgroup
         if (!d.slab || d.slab_elems < lds_spill_elems()) return (int)hipErrorInvalidValue;
@@ -437,11 +430,10 @@ int lds_launch(const BatchDesc& d, hipStream_t s) {
 }
 
 int lds_waves_override() {
-    static int v = -2;
-    if (v == -2) {
+    static const int v = [] {
         const char* e = getenv("PFH_LDS_WAVES");
-        v = e ? atoi(e) : -1;
-    }
+        return e ? atoi(e) : -1;
+    }();
     return v;
 }
 

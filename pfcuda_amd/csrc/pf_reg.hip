@@ -251,26 +251,18 @@ pf_reg_kernel(BatchDesc d) {
 // launch configuration tables
 // ---------------------------------------------------------------------------
 
-static int g_num_cu = 0;
-
-static int num_cu() {
-    if (g_num_cu == 0) {
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        int v = 0;
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
-        g_num_cu = v;
-    }
-    return g_num_cu;
-}
+static int num_cu() { return device_cu_count(); }
 
 template <typename T, int NCOL, int NRB, int LR, int MODE>
 static int launch_one(const BatchDesc& d, hipStream_t stream) {
     if (d.count <= 0) return 0;
     auto kern = pf_reg_kernel<T, NCOL, NRB, LR, MODE>;
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, NCOL * NRB, 0) != hipSuccess || occ <= 0) occ = 1;
-    int64_t grid = (int64_t)occ * num_cu();
+    static DevCache resident;   // resident workgroups on the device
+    int64_t grid = resident.get([&] {
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, NCOL * NRB, 0) != hipSuccess || occ <= 0) occ = 1;
+        return occ * num_cu();
+    });
     if (grid > d.count) grid = d.count;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NCOL * NRB), 0, stream, d);
     return (int)hipGetLastError();
@@ -378,11 +370,10 @@ int reg_grad_max_n(int dtype) {
 // lanes, whose rank-2 updates round as the reference's (rel 1e-12 on
 // 32 x 32).  PFH_F64_KERNEL=lanes|lds|bat forces one where it applies.
 static int f64_kernel_override() {
-    static int v = -2;
-    if (v == -2) {
+    static const int v = [] {
         const char* e = getenv("PFH_F64_KERNEL");
-        v = !e ? -1 : strcmp(e, "lanes") == 0 ? 0 : strcmp(e, "lds") == 0 ? 2 : strcmp(e, "bat") == 0 ? 3 : -1;
-    }
+        return !e ? -1 : strcmp(e, "lanes") == 0 ? 0 : strcmp(e, "lds") == 0 ? 2 : strcmp(e, "bat") == 0 ? 3 : -1;
+    }();
     return v;
 }
 
@@ -413,11 +404,10 @@ int launch_reg(int dtype, int mode, const BatchDesc& d, int maxn, hipStream_t s)
     if (dtype == DT_C128 && mode == MODE_SLOG && maxn > 64) {
         // complex128 slog: the batched blocked kernel above 64 (the lanes
         // kernel holds at most 128; PFH_C128_KERNEL=lanes keeps it there)
-        static int lanes_env = -1;
-        if (lanes_env < 0) {
+        static const int lanes_env = [] {
             const char* e = getenv("PFH_C128_KERNEL");
-            lanes_env = (e && strcmp(e, "lanes") == 0) ? 1 : 0;
-        }
+            return (e && strcmp(e, "lanes") == 0) ? 1 : 0;
+        }();
         const int tv = maxn <= 128 ? tune_value(PFH_TUNE_C128_BUCKET_128) : -1;
         const bool lanes = tv >= 0 ? tv == 0 : lanes_env == 1;
         if (d.bslab && !(lanes && maxn <= 128)) return launch_bat(DT_C128, d, maxn, s);
