@@ -63,7 +63,10 @@ int pfh_slog_pfaffian_workspace_size(int dtype, int64_t batch, int32_t n, int me
  * host-driven slog_pfaffian_lg (src/slog_pfaffian_lg.cu:151-211).  The log
  * output is double for every dtype.  The input is never modified.
  * `workspace` may be NULL, in which case the library uses a cached
- * per-device buffer grown with hipMalloc (not capture-safe). */
+ * per-device buffer grown with hipMalloc (not capture-safe).  Every such
+ * call on a device shares that buffer: calls that overlap (other streams,
+ * other host threads) race on it, so concurrent callers pass their own
+ * workspace.  Growing it synchronizes the device first. */
 int pfh_slog_pfaffian_f32(const float* A, int64_t batch, int32_t n, double* log_out, float* sign_out,
                           void* workspace, size_t workspace_bytes, hipStream_t stream);
 int pfh_slog_pfaffian_f64(const double* A, int64_t batch, int32_t n, double* log_out, double* sign_out,
