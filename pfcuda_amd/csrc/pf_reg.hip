@@ -135,6 +135,10 @@ pf_reg_kernel(BatchDesc d) {
                 if (rb == (f - LR) % NRB) Xs[c] = a.dget((f - LR) / NRB);
             }
             __syncthreads();   // B1
+            // logical positions k+1, k+2 (written only after B2): read now, so
+            // that their latency overlaps the search
+            const int p1 = pos[k + 1];
+            const int p2 = (k + 2 < n) ? pos[k + 2] : 0;
 
             // 2. pivot search over logical columns k+1 .. n-1: every wave
             //    reduces all columns with DPP (the largest magnitude, then
@@ -156,7 +160,7 @@ pf_reg_kernel(BatchDesc d) {
             int lq, q;
             if (wk == 0x7fffffff) {                // only NaN candidates: keep k+1
                 lq = k + 1;
-                q = __builtin_amdgcn_readfirstlane(pos[k + 1]);
+                q = __builtin_amdgcn_readfirstlane(p1);
             } else {
                 lq = wk >> 10;
                 q = wk & 1023;
@@ -164,8 +168,7 @@ pf_reg_kernel(BatchDesc d) {
             const T piv = Xs[q];
             if (is_zero(piv)) { dead = true; break; }
             const bool sw = (lq != k + 1);
-            const int p1 = pos[k + 1];
-            const int f_next = (k + 2 < n) ? ((lq == k + 2) ? p1 : pos[k + 2]) : 0;
+            const int f_next = (k + 2 < n) ? ((lq == k + 2) ? p1 : p2) : 0;
 
             // 3. row q -> (X, Y)
             bool ownq;
