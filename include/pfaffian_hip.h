@@ -134,7 +134,10 @@ int pfh_pfaffian_cpu_f64(double* A, int32_t n, double* out);
 #define PFH_TUNE_TRAIL_SWEEP 6      /* block sweep (gradient) trailing update: same values */
 #define PFH_TUNE_PANEL_LAUNCH 7     /* blocked panel: 0 plain launch, 1 cooperative launch */
 #define PFH_TUNE_GRAD_BLOCKED 8     /* gradient past the register kernel: 0 unblocked sweep, 1 blocked sweep */
-#define PFH_TUNE_NKEYS 9
+#define PFH_TUNE_PANEL_SPIN 9       /* blocked panel: reads of the candidate records a poll makes before the
+                                       panel sets its error flag (output NaN), 1 .. 4194304 (the default);
+                                       0 sets it at once.  For tests of that path, not a tuning choice */
+#define PFH_TUNE_NKEYS 10
 
 /* Sets a tuning key (value -1: the built-in choice); PFH_EINVAL for an
  * unknown key or a value outside the key's list.  Takes effect for calls

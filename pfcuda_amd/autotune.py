@@ -138,7 +138,7 @@ def tune(out: str | None = None, quick: bool = False, verbose: bool = True, keys
     n_big = 1024 if quick else 2048
     reps = 2 if quick else 5
     choices, timings = {}, {}
-    for name in (keys or list(TUNE_KEYS)):
+    for name in (keys or list(CANDIDATES)):   # the run-time choices (not the panel's spin limit)
         set_tuning(name, -1)
         run = _bucket_workload(torch, name, count, dev) if name in BUCKETS else _single_workload(torch, name, n_big, dev)
         ref = run()
@@ -174,7 +174,7 @@ def tune(out: str | None = None, quick: bool = False, verbose: bool = True, keys
 def apply(choices: dict) -> None:
     """Set every key named in `choices` (unknown names are an error)."""
     for name, v in choices.items():
-        if name not in TUNE_KEYS:
+        if name not in CANDIDATES:
             raise KeyError(f"unknown tuning key {name!r}")
         set_tuning(name, int(v))
 

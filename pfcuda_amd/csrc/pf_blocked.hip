@@ -95,6 +95,7 @@ struct BPArgs {
     BPState<T>* st;
     int64_t n;
     int nb;
+    int spin;        // the cooperative panel's poll: reads of the records before its error flag
 };
 
 // Element (r, c) of the stale matrix seen through perm, from the upper triangle.
@@ -450,25 +451,25 @@ constexpr bool panel_cand() { return !is_cplx<T>::value; }   // the winner's ent
 // up within the spin limit sets err (the output becomes NaN); every record
 // word ever written decodes to indices in range, so nothing faults then.
 template <bool NF>
-__device__ __forceinline__ void bc_poll(const unsigned long long* rec_all, int e, int* err, int* s_w, int* s_p,
-                                        int* s_sp) {
+__device__ __forceinline__ void bc_poll(const unsigned long long* rec_all, int e, int limit, int* err, int* s_w,
+                                        int* s_p, int* s_sp) {
     if (threadIdx.x < 64) {
         const int g = threadIdx.x;
         const int G = gridDim.x;
         const unsigned long long pb = bc_par(e);
         const unsigned long long* rec = rec_all + ((int64_t)(e % 3) * BC_MAXG + (g < G ? g : 0)) * 2;
         unsigned long long wa = BC_NONE_A, wb = BC_NONE_B;
-        for (int spins = 0;; ++spins) {
+        for (int reads = 0;; ++reads) {
+            if (reads >= limit) {   // seconds by default: the grid is co-resident, so this is a failure
+                if (g == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
             if (g < G) {
                 wa = __hip_atomic_load(&rec[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 wb = __hip_atomic_load(&rec[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             const bool ok = g >= G || ((((wa ^ pb) | (wb ^ pb)) & BC_PAR) == 0);
             if (__all(ok)) break;
-            if (spins > (1 << 22)) {   // seconds: the grid is co-resident, so this is a failure
-                if (g == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
             __builtin_amdgcn_s_sleep(1);
         }
         if (g >= G) { wa = BC_NONE_A; wb = BC_NONE_B; }
@@ -602,7 +603,7 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         const bool has_next = i + 1 < np;       // row k+2 is this panel's next pivot row
         const int sa = Qpan[k + 1 - k0];
         const int sb = has_next ? Qpan[k + 2 - k0] : -1;
-        bc_poll<NF>(co.rec, e, &st->err, &s_w, &s_p, &s_sp);
+        bc_poll<NF>(co.rec, e, a.spin, &st->err, &s_w, &s_p, &s_sp);
         const int wb = s_w;
         int p, sp;
         if (wb >= 0) { p = s_p; sp = s_sp; }
@@ -1163,6 +1164,7 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
     a.st = (BPState<T>*)(base + L.st);
     a.n = n;
     a.nb = nb;
+    a.spin = tune_value(PFH_TUNE_PANEL_SPIN) >= 0 ? tune_value(PFH_TUNE_PANEL_SPIN) : (1 << 22);
     BPCoop<T> co{};
     co.rec = (unsigned long long*)(base + L.rec);
     co.cand = (T*)(base + L.cand);
@@ -1708,6 +1710,7 @@ int blocked_grad(const T* A, int64_t n, void* ws, double* log_out, T* phase_out,
     a.st = (BPState<T>*)(base + L.bp.st);
     a.n = n;
     a.nb = NB;
+    a.spin = tune_value(PFH_TUNE_PANEL_SPIN) >= 0 ? tune_value(PFH_TUNE_PANEL_SPIN) : (1 << 22);
     BPCoop<T> co{};
     co.rec = (unsigned long long*)(base + L.bp.rec);
     co.cand = (T*)(base + L.bp.cand);

@@ -40,6 +40,7 @@ bool tune_value_ok(int key, int v) {
         case PFH_TUNE_TRAIL_SWEEP:
         case PFH_TUNE_PANEL_LAUNCH:
         case PFH_TUNE_GRAD_BLOCKED: return v == 0 || v == 1;
+        case PFH_TUNE_PANEL_SPIN: return v >= 0 && v <= (1 << 22);
     }
     return false;
 }

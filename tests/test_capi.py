@@ -105,11 +105,13 @@ def test_set_get_tuning(lib):
         assert lib.pfh_set_tuning(4, 2) == -1            # not a value of this key
         assert lib.pfh_set_tuning(0, 1) == -1            # 1 is no kernel of an f64 bucket
         assert lib.pfh_set_tuning(0, 2) == 0
-        assert lib.pfh_set_tuning(9, 0) == -1 and lib.pfh_set_tuning(-1, 0) == -1
-        assert lib.pfh_get_tuning(9, ctypes.byref(v)) == -1
+        assert lib.pfh_set_tuning(9, 0) == 0 and lib.pfh_set_tuning(9, 1 << 22) == 0   # panel spin limit
+        assert lib.pfh_set_tuning(9, (1 << 22) + 1) == -1 and lib.pfh_set_tuning(9, -2) == -1
+        assert lib.pfh_set_tuning(10, 0) == -1 and lib.pfh_set_tuning(-1, 0) == -1
+        assert lib.pfh_get_tuning(10, ctypes.byref(v)) == -1
         assert lib.pfh_get_tuning(0, None) == -1
     finally:
-        for k in range(9):
+        for k in range(10):
             lib.pfh_set_tuning(k, -1)
 
 

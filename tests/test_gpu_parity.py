@@ -183,6 +183,39 @@ def test_ragged_above_256_and_beyond_max_n(pf, cuda, dtype):
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
+def test_panel_poll_limit_gives_nan_then_recovers(pf, cuda, dtype):
+    # The cooperative panel's wait for the other blocks' candidate records is
+    # bounded: past its limit it sets the panel's error flag, the output is
+    # NaN, and nothing hangs.  The limit is forced to zero from the host
+    # (polls give up before reading), for the solve and for the gradient's
+    # block sweep; the next calls, at the default limit, are exact again.
+    from pfcuda_amd import _lib
+    from pfcuda_amd.adjoint import slog_grad
+    rng = np.random.default_rng(77)
+    A = oracle.skew_gaussian(rng, 520, complex_=dtype.is_complex)
+    At = torch.from_numpy(A).to(cuda)
+    l0, p0 = oracle.slog(A)
+    spin, gb = _lib.TUNE_KEYS["panel_spin"], _lib.TUNE_KEYS["grad_blocked"]
+    try:
+        _lib.check(_lib.lib.pfh_set_tuning(gb, 1), "grad_blocked")
+        _lib.check(_lib.lib.pfh_set_tuning(spin, 0), "panel_spin")
+        la, ph = pf.slog_pfaffian(At, pf.PFH_METHOD_BLOCKED)
+        assert math.isnan(float(la)) and torch.isnan(torch.as_tensor(ph)).all()
+        G = slog_grad(At)
+        assert torch.isnan(G).all()
+    finally:
+        _lib.lib.pfh_set_tuning(spin, -1)
+    try:
+        la, ph = pf.slog_pfaffian(At, pf.PFH_METHOD_BLOCKED)
+        assert abs(float(la) - l0) < 1e-9 and abs(complex(ph) - complex(p0)) < 1e-9
+        G = slog_grad(At).cpu().numpy()
+        Gi = 0.5 * np.linalg.inv(A).T
+        assert np.abs(G - Gi).max() <= 1e-9 * np.abs(Gi).max()
+    finally:
+        _lib.lib.pfh_set_tuning(gb, -1)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
 @pytest.mark.parametrize("n", [300, 512])
 def test_unblocked_large_vs_oracle(pf, cuda, dtype, n):
     rng = np.random.default_rng(n)
