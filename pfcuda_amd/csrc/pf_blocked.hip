@@ -1320,16 +1320,28 @@ __global__ void __launch_bounds__(1024) gj_inv(const T* __restrict__ W, const in
         const T xrr = xr[par][r], yrr = yr[par][r];
         const bool pr = r == k || r == k + 1;
         const T ninv = -inv;
+        // (a thread's entries share its row, so the pivot rows' form is a
+        // branch taken by the one wave that holds rows k, k+1; every read
+        // above is issued before it)
+        if (!pr) {
 #pragma unroll
-        for (int i = 0; i < EPT; ++i) {
-            const int c = cq + TPR * i;
-            const bool pc = c == k || c == k + 1;
-            const T gen = v[i] + (yrr * xc[i] - xrr * yc[i]) * inv;
-            const T rowv = psel(r == k, yc[i] * inv, -(xc[i] * inv));
-            const T colv = psel(c == k, -(yrr * inv), xrr * inv);
-            const T pp = psel(r == c, zero<T>(), psel(r == k, ninv, inv));
-            const T u = psel(pr, psel(pc, pp, rowv), psel(pc, colv, gen));
-            v[i] = psel(r < m && c < m, u, v[i]);
+            for (int i = 0; i < EPT; ++i) {
+                const int c = cq + TPR * i;
+                const T gen = v[i] + (yrr * xc[i] - xrr * yc[i]) * inv;
+                const T colv = psel(c == k, -(yrr * inv), xrr * inv);
+                const T u = psel(c == k || c == k + 1, colv, gen);
+                v[i] = psel(r < m && c < m, u, v[i]);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < EPT; ++i) {
+                const int c = cq + TPR * i;
+                const bool pc = c == k || c == k + 1;
+                const T rowv = psel(r == k, yc[i] * inv, -(xc[i] * inv));
+                const T pp = psel(r == c, zero<T>(), psel(r == k, ninv, inv));
+                const T u = psel(pc, pp, rowv);
+                v[i] = psel(r < m && c < m, u, v[i]);
+            }
         }
     }
 #pragma unroll
