@@ -1348,12 +1348,15 @@ __global__ void __launch_bounds__(1024) gj_inv(const T* __restrict__ W, const in
     for (int i = 0; i < EPT; ++i) Ai[r * M + cq + TPR * i] = v[i];
 }
 
+constexpr int GJ_CB = 16;   // gj_rows: columns per block (16 row groups of 256 / 16 threads)
+
 template <typename T, int NB>
 __global__ void __launch_bounds__(256) gj_rows(const T* __restrict__ W, const int* __restrict__ perm, int64_t n,
                                                int k0, int np, const BPState<T>* st, const T* __restrict__ Ai,
                                                T* __restrict__ S, T* __restrict__ Z, const int* __restrict__ gp_old,
                                                int* __restrict__ gp_new) {
-    constexpr int M = 2 * NB, CB = 32, HPT = M / 8;
+    constexpr int M = 2 * NB, CB = GJ_CB, HG = 256 / CB, HPT = M / HG;
+    static_assert(256 % CB == 0 && M % HG == 0, "row groups tile the block");
     __shared__ T As[M][M + 1];
     __shared__ T Sc[M][CB + 1];
     if (st->dead) return;
@@ -1377,18 +1380,18 @@ __global__ void __launch_bounds__(256) gj_rows(const T* __restrict__ W, const in
     for (int i = 0; i < HPT; ++i) acc[i] = zero<T>();
     if (gj_in(x, k0, m)) {
 #pragma unroll
-        for (int i = 0; i < HPT; ++i) acc[i] = -As[hg + 8 * i][x - k0];
+        for (int i = 0; i < HPT; ++i) acc[i] = -As[hg + HG * i][x - k0];
     } else {
         for (int hp = 0; hp < m; ++hp) {
             const T sv = Sc[hp][cx];
 #pragma unroll
-            for (int i = 0; i < HPT; ++i) acc[i] = acc[i] + As[hg + 8 * i][hp] * sv;
+            for (int i = 0; i < HPT; ++i) acc[i] = acc[i] + As[hg + HG * i][hp] * sv;
         }
     }
     if (x < n) {
 #pragma unroll
         for (int i = 0; i < HPT; ++i) {
-            const int h = hg + 8 * i;
+            const int h = hg + HG * i;
             if (h < m) { S[(int64_t)h * n + x] = Sc[h][cx]; Z[(int64_t)h * n + x] = acc[i]; }
         }
     }
@@ -1747,7 +1750,7 @@ int blocked_grad(const T* A, int64_t n, void* ws, double* log_out, T* phase_out,
         if (e != hipSuccess) return (int)e;
         epoch += np;
         hipLaunchKernelGGL((gj_inv<T, NB>), dim3(1), dim3(1024), 0, s, W[cur], a.perm, n, (int)k0, np, a.st, Ai);
-        hipLaunchKernelGGL((gj_rows<T, NB>), dim3((unsigned)((n + 31) / 32)), dim3(256), 0, s, W[cur], a.perm, n,
+        hipLaunchKernelGGL((gj_rows<T, NB>), dim3((unsigned)((n + GJ_CB - 1) / GJ_CB)), dim3(256), 0, s, W[cur], a.perm, n,
                            (int)k0, np, a.st, Ai, S, Z, gp[gc], gp[gc ^ 1]);
         if constexpr (std::is_same<T, double>::value || std::is_same<T, c128>::value) {
             if (trail_staged(PFH_TUNE_TRAIL_SWEEP, true)) {
