@@ -25,6 +25,7 @@ extern "C" {
 #define PFH_EINVAL (-1)      /* bad size, count or pointer */
 #define PFH_EWORKSPACE (-2)  /* workspace too small */
 #define PFH_ETOOBIG (-3)     /* n beyond what the entry point supports */
+#define PFH_ETIMEOUT (-4)    /* a blocked panel's blocks were not resident together (see pfh_stream_status) */
 
 /* dtype codes */
 #define PFH_F32 0
@@ -43,6 +44,16 @@ const char* pfh_status_string(int status);
 
 /* Largest n that pfh_pfaffian_* accepts (64). */
 int pfh_pfaffian_max_n(void);
+
+/* Waits for the work enqueued on `stream`, then returns PFH_ETIMEOUT if a
+ * blocked panel on the current device gave up waiting for its blocks since
+ * the last call (the outputs of the call it belonged to are NaN), and clears
+ * that report; else 0, or a hipError_t.  The blocked panel assumes its
+ * <= 64 blocks become resident together; work from other streams holding
+ * the CUs for seconds can defeat that, and this is how a caller learns of
+ * it.  No counterpart in the reference, whose large path synchronizes with
+ * the host at every pivot (src/slog_pfaffian_lg.cu:178). */
+int pfh_stream_status(hipStream_t stream);
 
 /* Pf of `batch` matrices of size n (product form, the input dtype).
  * Replaces the XLA FFI handlers pfaffian_{f32,f64,c64,c128}
@@ -110,6 +121,19 @@ int pfh_slog_pfaffian_grad(int dtype, const void* A, int64_t batch, int32_t n, d
                            void* phase_out, void* grad_out, void* workspace, size_t workspace_bytes,
                            hipStream_t stream);
 
+/* Host-pointer forms: one row-major n x n matrix in HOST memory, copied to
+ * the current device, computed, and the result copied back; the call
+ * returns when the result is in *h_out (synchronous, on the null stream),
+ * and frees every device buffer it made.  Replace the reference's
+ * host-pointer templates pfaffian<T>(h_A, n) (src/pfaffian.cu:4-40) and
+ * slog_pfaffian<T>(h_A, n, h_log_abs, h_phase) (src/slog_pfaffian.cu:4-32),
+ * which read column-major input and leak their device buffers (slog).
+ * pfh_pfaffian_host takes n <= 64 (PFH_ETOOBIG past it) and writes the
+ * input dtype; pfh_slog_pfaffian_host takes any n and writes log|Pf|
+ * (double) and the sign / phase (input dtype). */
+int pfh_pfaffian_host(int dtype, const void* h_A, int32_t n, void* h_out);
+int pfh_slog_pfaffian_host(int dtype, const void* h_A, int32_t n, double* h_log, void* h_phase);
+
 /* Pfaffian of one row-major n x n float64 matrix in HOST memory, on the
  * CPU (FP64, one thread, product form), working in place on A: the
  * counterpart of the reference's public pfaffian_cpu (pfcuda/cpp_api.py:3-4,
@@ -132,7 +156,7 @@ int pfh_pfaffian_cpu_f64(double* A, int32_t n, double* out);
 #define PFH_TUNE_TRAIL_F64 4        /* blocked factorization, f64 trailing update: 0 register form, 1 LDS-staged */
 #define PFH_TUNE_TRAIL_C128 5       /* blocked factorization, c128 trailing update: same values */
 #define PFH_TUNE_TRAIL_SWEEP 6      /* block sweep (gradient) trailing update: same values */
-#define PFH_TUNE_PANEL_LAUNCH 7     /* blocked panel: 0 plain launch, 1 cooperative launch */
+#define PFH_TUNE_PANEL_LAUNCH 7     /* blocked panel launch form: 0 plain launch (the only form) */
 #define PFH_TUNE_GRAD_BLOCKED 8     /* gradient past the register kernel: 0 unblocked sweep, 1 blocked sweep */
 #define PFH_TUNE_PANEL_SPIN 9       /* blocked panel: reads of the candidate records a poll makes before the
                                        panel sets its error flag (output NaN), 1 .. 4194304 (the default);

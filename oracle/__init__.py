@@ -12,6 +12,9 @@ Contents, each citing the reference function it restates:
   reference's pivot rule bit for bit (pinned against ``oracle/_ref``).
 * ``pf_f64_ld`` / ``pf_c128_ld`` — the same elimination in x87 long double,
   the extended-precision "truth" for the mixed-precision accuracy checks.
+* ``pf_f64_mt`` / ``pf_c128_mt`` — the same elimination with each pair
+  step's column loops split over host threads, bitwise equal to the above
+  (for the N = 4096 / 8192 single-matrix checks).
 * ``slog_ragged_f64`` / ``slog_ragged_c128`` — threaded batch driver (one
   matrix per task) used as the CPU baseline.
 * ``pfaffian_py`` — NumPy restatement of ``pfaffian_py``
@@ -61,6 +64,8 @@ def _lib():
         lib.oracle_pf_f64_ld.argtypes = [_dp, ctypes.c_int, _dp]
         lib.oracle_pf_c128.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         lib.oracle_pf_c128_ld.argtypes = [_dp, ctypes.c_int, _dp]
+        lib.oracle_pf_f64_mt.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp]
+        lib.oracle_pf_c128_mt.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp]
         for name in ("oracle_slog_ragged_f64", "oracle_slog_ragged_c128"):
             getattr(lib, name).argtypes = [_dp, _i64p, _i32p, ctypes.c_int64, ctypes.c_int, _dp, _dp]
             getattr(lib, name).restype = None
@@ -119,12 +124,44 @@ def pf_c128_ld(A: np.ndarray):
     return complex(out[0], out[1]), float(out[2]), complex(out[3], out[4]), int(info)
 
 
-def slog(A: np.ndarray):
-    """(logabs, sign-or-phase) for real or complex input via the C++ oracle."""
+def default_threads() -> int:
+    """Host threads for the threaded oracle: the cores this process may use, at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def pf_f64_mt(A: np.ndarray, threads: int | None = None):
+    """pf_f64 with each pair step's column loops split over host threads.
+
+    Bitwise equal to ``pf_f64(A)`` (each column keeps its operation order;
+    tests/test_oracle.py checks it), for the N = 4096 / 8192 checks."""
+    a = np.ascontiguousarray(A, dtype=np.float64)
+    out = np.zeros(3)
+    info = _lib().oracle_pf_f64_mt(_as_dp(a), a.shape[0], threads or default_threads(), _as_dp(out))
+    return float(out[0]), float(out[1]), float(out[2]), int(info)
+
+
+def pf_c128_mt(A: np.ndarray, threads: int | None = None):
+    """pf_c128 with threaded column loops, bitwise equal to ``pf_c128(A)``."""
+    a = np.ascontiguousarray(A, dtype=np.complex128)
+    out = np.zeros(5)
+    info = _lib().oracle_pf_c128_mt(_as_dp(a.view(np.float64)), a.shape[0], threads or default_threads(),
+                                    _as_dp(out))
+    return complex(out[0], out[1]), float(out[2]), complex(out[3], out[4]), int(info)
+
+
+def slog(A: np.ndarray, threads: int | None = None):
+    """(logabs, sign-or-phase) for real or complex input via the C++ oracle.
+
+    Matrices of 512 and more run on the threaded form (bitwise the same)."""
+    big = A.shape[-1] >= 512
     if np.iscomplexobj(A):
-        _, la, ph, _ = pf_c128(A)
+        _, la, ph, _ = pf_c128_mt(A, threads) if big else pf_c128(A)
         return la, ph
-    _, la, s, _ = pf_f64(A)
+    _, la, s, _ = pf_f64_mt(A, threads) if big else pf_f64(A)
     return la, s
 
 

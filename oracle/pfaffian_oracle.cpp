@@ -32,6 +32,7 @@
 #include <vector>
 #include <atomic>
 #include <algorithm>
+#include <functional>
 
 namespace {
 
@@ -69,8 +70,114 @@ template <typename S> struct PfResult {
 // Row-major element access, as RM() in include/pfaffian_cpu.h:3.
 #define ORM(A, n, i, j) ((A)[(size_t)(i) * (size_t)(n) + (size_t)(j)])
 
+// Row `row` (k or k+1) at columns [c0, c1) gets the corrections of the
+// stored pairs r < k (src/pfaffian_cpu.cpp:32-45 and :85-98):
+//   A[row][c] += A[r][c] * A[r+1][row] - A[r+1][c] * A[r][row].
+// Every column is independent, so a column range can go to its own thread
+// with each element's operations in the same order as one thread's.
+// Complex values are multiplied as (ac - bd, ad + bc), the value the
+// compiler's complex product returns for finite operands, written out so
+// the loop avoids its NaN-recovery call.
 template <typename S>
-PfResult<S> pfaffian_left_looking(S* A, int n, bool stale) {
+inline void lazy_update(S* A, int n, int k, int row, int c0, int c1) {
+    S* v = &ORM(A, n, row, 0);
+    for (int r = 0; r < k; r += 2) {
+        const S* s0 = &ORM(A, n, r, 0);      // x_r / a_r, scaled
+        const S* s1 = &ORM(A, n, r + 1, 0);  // y_r
+        const S f0 = s1[row], f1 = s0[row];
+        if constexpr (std::is_floating_point<S>::value) {
+            for (int c = c0; c < c1; ++c) v[c] += s0[c] * f0 - s1[c] * f1;
+        } else {
+            using R = typename Real<S>::type;
+            const R f0r = f0.real(), f0i = f0.imag(), f1r = f1.real(), f1i = f1.imag();
+            R* vp = reinterpret_cast<R*>(v);
+            const R* p0 = reinterpret_cast<const R*>(s0);
+            const R* p1 = reinterpret_cast<const R*>(s1);
+            for (int c = c0; c < c1; ++c) {
+                const R ar = p0[2 * c], ai = p0[2 * c + 1], br = p1[2 * c], bi = p1[2 * c + 1];
+                const R t1r = ar * f0r - ai * f0i, t1i = ar * f0i + ai * f0r;
+                const R t2r = br * f1r - bi * f1i, t2i = br * f1i + bi * f1r;
+                vp[2 * c] = vp[2 * c] + (t1r - t2r);
+                vp[2 * c + 1] = vp[2 * c + 1] + (t1i - t2i);
+            }
+        }
+    }
+}
+
+template <typename S>
+inline void scale_row(S* v, S inv, int c0, int c1) {
+    if constexpr (std::is_floating_point<S>::value) {
+        for (int c = c0; c < c1; ++c) v[c] *= inv;
+    } else {
+        using R = typename Real<S>::type;
+        const R ir = inv.real(), ii = inv.imag();
+        R* vp = reinterpret_cast<R*>(v);
+        for (int c = c0; c < c1; ++c) {
+            const R a = vp[2 * c], b = vp[2 * c + 1];
+            vp[2 * c] = a * ir - b * ii;
+            vp[2 * c + 1] = a * ii + b * ir;
+        }
+    }
+}
+
+// Runs fn(c0, c1) over [a, b) on the calling thread.
+struct SerialFor {
+    template <typename F> void operator()(int a, int b, F&& fn) const { if (a < b) fn(a, b); }
+};
+
+// A fixed set of threads that split a column range [a, b) into contiguous
+// chunks, one per thread, and return when every chunk is done.  Each column
+// is handled by exactly one thread with the same operations as the serial
+// loop, so a run on this pool is bitwise equal to a serial run.
+class ColumnPool {
+public:
+    explicit ColumnPool(int threads) : T_(threads < 1 ? 1 : threads) {
+        for (int id = 1; id < T_; ++id) th_.emplace_back([this, id] { worker(id); });
+    }
+    ~ColumnPool() {
+        stop_.store(true, std::memory_order_relaxed);
+        gen_.fetch_add(1, std::memory_order_release);
+        for (auto& t : th_) t.join();
+    }
+    template <typename F> void operator()(int a, int b, F&& fn) {
+        if (a >= b) return;
+        if (T_ == 1 || b - a < 256) { fn(a, b); return; }
+        std::function<void(int, int)> f = fn;
+        job_ = &f; a_ = a; b_ = b;
+        done_.store(0, std::memory_order_relaxed);
+        gen_.fetch_add(1, std::memory_order_release);
+        run(0);
+        while (done_.load(std::memory_order_acquire) != T_ - 1) pause();
+    }
+private:
+    static void pause() { std::this_thread::yield(); }
+    void run(int id) {
+        const int len = b_ - a_;
+        const int per = ((len + T_ - 1) / T_ + 7) & ~7;   // chunks of whole 64-byte lines
+        const int c0 = a_ + id * per, c1 = std::min(b_, c0 + per);
+        if (c0 < c1) (*job_)(c0, c1);
+    }
+    void worker(int id) {
+        int seen = 0;
+        for (;;) {
+            int g;
+            while ((g = gen_.load(std::memory_order_acquire)) == seen) pause();
+            seen = g;
+            if (stop_.load(std::memory_order_relaxed)) return;
+            run(id);
+            done_.fetch_add(1, std::memory_order_release);
+        }
+    }
+    int T_;
+    std::vector<std::thread> th_;
+    std::atomic<int> gen_{0}, done_{0};
+    std::atomic<bool> stop_{false};
+    const std::function<void(int, int)>* job_ = nullptr;
+    int a_ = 0, b_ = 0;
+};
+
+template <typename S, typename For = SerialFor>
+PfResult<S> pfaffian_left_looking(S* A, int n, bool stale, For&& pfor = For()) {
     using R = typename Real<S>::type;
     PfResult<S> out{S(0), -INFINITY, S(0), 0};
     if (n == 0 || (n & 1)) return out;            // src/pfaffian_cpu.cpp:8-9
@@ -81,17 +188,11 @@ PfResult<S> pfaffian_left_looking(S* A, int n, bool stale) {
 
     for (int k = 0; k < n - 1; k += 2) {
         S* v = &ORM(A, n, k, 0);       // row k, the pivot row
-        S* w = &ORM(A, n, k + 1, 0);   // row k+1
         int kp = k + 1;
         R kp_val = mag(v[kp]);         // stale start value (src/pfaffian_cpu.cpp:29-30)
 
         // Lazy update of row k from the stored pairs (src/pfaffian_cpu.cpp:32-45).
-        for (int r = 0; r < k; r += 2) {
-            const S* s0 = &ORM(A, n, r, 0);      // x_r / a_r, scaled
-            const S* s1 = &ORM(A, n, r + 1, 0);  // y_r
-            const S f0 = s1[k], f1 = s0[k];
-            for (int c = k + 1; c < n; ++c) v[c] += s0[c] * f0 - s1[c] * f1;
-        }
+        pfor(k + 1, n, [&](int c0, int c1) { lazy_update<S>(A, n, k, k, c0, c1); });
         if (!stale) kp_val = mag(v[kp]);
         for (int c = k + 2; c < n; ++c) {      // src/pfaffian_cpu.cpp:47-52
             R cv = mag(v[c]);
@@ -126,13 +227,10 @@ PfResult<S> pfaffian_left_looking(S* A, int n, bool stale) {
         const S inv = S(1) / pivot;
 
         // Lazy update of row k+1 (src/pfaffian_cpu.cpp:85-98), then scale row k (:100-102).
-        for (int r = 0; r < k; r += 2) {
-            const S* s0 = &ORM(A, n, r, 0);
-            const S* s1 = &ORM(A, n, r + 1, 0);
-            const S f0 = s1[k + 1], f1 = s0[k + 1];
-            for (int c = k + 2; c < n; ++c) w[c] += s0[c] * f0 - s1[c] * f1;
-        }
-        for (int c = k + 2; c < n; ++c) v[c] *= inv;
+        pfor(k + 2, n, [&](int c0, int c1) {
+            lazy_update<S>(A, n, k, k + 1, c0, c1);
+            scale_row<S>(v, inv, c0, c1);
+        });
     }
     out.prod = result;
     out.logabs = logsum.value();
@@ -223,6 +321,27 @@ int oracle_pf_c128_ld(const double* A, int n, double* out5) {
     PfResult<C> r = pfaffian_left_looking<C>(w.data(), n, false);
     out5[0] = (double)r.prod.real(); out5[1] = (double)r.prod.imag(); out5[2] = (double)r.logabs;
     out5[3] = (double)r.phase.real(); out5[4] = (double)r.phase.imag();
+    return r.info;
+}
+
+// The same eliminations with each pair step's column loops split across
+// `threads` host threads (bitwise equal to the calls above; for the large
+// single-matrix checks).
+int oracle_pf_f64_mt(const double* A, int n, int threads, double* out3) {
+    std::vector<double> w(A, A + (size_t)n * n);
+    ColumnPool pool(threads);
+    PfResult<double> r = pfaffian_left_looking<double>(w.data(), n, false, pool);
+    out3[0] = r.prod; out3[1] = r.logabs; out3[2] = r.phase;
+    return r.info;
+}
+
+int oracle_pf_c128_mt(const double* A, int n, int threads, double* out5) {
+    using C = std::complex<double>;
+    std::vector<C> w(reinterpret_cast<const C*>(A), reinterpret_cast<const C*>(A) + (size_t)n * n);
+    ColumnPool pool(threads);
+    PfResult<C> r = pfaffian_left_looking<C>(w.data(), n, false, pool);
+    out5[0] = r.prod.real(); out5[1] = r.prod.imag(); out5[2] = r.logabs;
+    out5[3] = r.phase.real(); out5[4] = r.phase.imag();
     return r.info;
 }
 

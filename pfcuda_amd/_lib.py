@@ -55,6 +55,9 @@ def _load():
     lib.pfh_slog_pfaffian_grad_workspace_size.argtypes = [ctypes.c_int, _i64, _i32, ctypes.POINTER(_sz)]
     lib.pfh_slog_pfaffian_grad.argtypes = [ctypes.c_int, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _sz, _vp]
     lib.pfh_pfaffian_cpu_f64.argtypes = [_vp, _i32, ctypes.POINTER(ctypes.c_double)]
+    lib.pfh_pfaffian_host.argtypes = [ctypes.c_int, _vp, _i32, _vp]
+    lib.pfh_slog_pfaffian_host.argtypes = [ctypes.c_int, _vp, _i32, _vp, _vp]
+    lib.pfh_stream_status.argtypes = [_vp]
     lib.pfh_set_tuning.argtypes = [ctypes.c_int, ctypes.c_int]
     lib.pfh_get_tuning.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     return lib

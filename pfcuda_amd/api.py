@@ -358,6 +358,10 @@ def _ragged_kernel(data: torch.Tensor, offsets: torch.Tensor, sizes: torch.Tenso
     return logabs, phase
 
 
+def _on_host(x) -> bool:
+    return not (isinstance(x, torch.Tensor) and x.is_cuda)
+
+
 def slog_pfaffian_ragged(data, offsets, sizes, max_n: int | None = None):
     """(log|Pf|, sign/phase) of every matrix of a packed ragged batch.
 
@@ -365,12 +369,77 @@ def slog_pfaffian_ragged(data, offsets, sizes, max_n: int | None = None):
     sizes[i] x sizes[i] row-major slab at element offsets[i].  One call, one
     persistent launch per size bucket, no host synchronisation once
     ``max_n`` is given (without it, the largest size is read back first).
+    Offsets and sizes given in host memory are checked against ``data``
+    here (ValueError); device tensors are taken as they are, so that the
+    call stays free of host synchronisation.  A matrix larger than
+    ``max_n`` gives NaN without disturbing the others.
     """
     data = _to_device(data).contiguous()
     dev = data.device
+    if _on_host(offsets) and _on_host(sizes):
+        o = np.asarray(torch.as_tensor(offsets).cpu(), dtype=np.int64).reshape(-1)
+        z = np.asarray(torch.as_tensor(sizes).cpu(), dtype=np.int64).reshape(-1)
+        if o.shape != z.shape:
+            raise ValueError(f"offsets ({o.size}) and sizes ({z.size}) differ in length")
+        if z.size:
+            if (z < 0).any() or (o < 0).any():
+                raise ValueError("offsets and sizes must be non-negative")
+            if (z > 2**31 - 1).any():
+                raise ValueError("a size does not fit int32")
+            if int((o + z * z).max()) > data.numel():
+                bad = int(np.argmax(o + z * z > data.numel()))
+                raise ValueError(f"matrix {bad} (offset {o[bad]}, size {z[bad]}) runs past the end of "
+                                 f"data ({data.numel()} elements)")
     offsets = torch.as_tensor(offsets, dtype=torch.int64).to(dev).contiguous()
     sizes_t = torch.as_tensor(sizes, dtype=torch.int32)
     if max_n is None:
         max_n = int(sizes_t.max().item()) if sizes_t.numel() else 0
     sizes_t = sizes_t.to(dev).contiguous()
     return _ragged_op(data, offsets, sizes_t, int(max_n))
+
+
+# ---------------------------------------------------------------------------
+# host-pointer forms and the stream status
+# ---------------------------------------------------------------------------
+
+_NP_DTYPES = {np.dtype(np.float32): _lib.PFH_F32, np.dtype(np.float64): _lib.PFH_F64,
+              np.dtype(np.complex64): _lib.PFH_C64, np.dtype(np.complex128): _lib.PFH_C128}
+
+
+def _host_matrix(A) -> Tuple[np.ndarray, int]:
+    a = np.ascontiguousarray(A)
+    if a.dtype not in _NP_DTYPES:
+        raise TypeError(f"unsupported dtype {a.dtype}; use float32/64 or complex64/128")
+    if a.ndim != 2 or a.shape[0] != a.shape[1]:
+        raise ValueError(f"expected one square matrix; received shape {a.shape}")
+    return a, _NP_DTYPES[a.dtype]
+
+
+def pfaffian_host(A):
+    """Pf of one n x n matrix in host memory (NumPy), through the
+    synchronous host-pointer entry point (as the reference's
+    pfaffian<T>(h_A, n), src/pfaffian.cu:4-40; n <= 64)."""
+    a, dt = _host_matrix(A)
+    out = np.zeros(1, dtype=a.dtype)
+    check(lib.pfh_pfaffian_host(dt, a.ctypes.data, a.shape[0], out.ctypes.data), "pfaffian_host")
+    return out[0]
+
+
+def slog_pfaffian_host(A):
+    """(log|Pf|, sign or phase) of one matrix in host memory, through the
+    synchronous host-pointer entry point (as the reference's
+    slog_pfaffian<T>(h_A, n, ...), src/slog_pfaffian.cu:4-32)."""
+    a, dt = _host_matrix(A)
+    la = np.zeros(1, dtype=np.float64)
+    ph = np.zeros(1, dtype=a.dtype)
+    check(lib.pfh_slog_pfaffian_host(dt, a.ctypes.data, a.shape[0], la.ctypes.data, ph.ctypes.data),
+          "slog_pfaffian_host")
+    return float(la[0]), ph[0]
+
+
+def check_status(device=None) -> None:
+    """Wait for the current stream and raise if a blocked panel on this
+    device gave up waiting for its blocks since the last check (the results
+    of that call are NaN)."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    check(lib.pfh_stream_status(torch.cuda.current_stream(dev).cuda_stream), "stream status")

@@ -32,7 +32,7 @@ CANDIDATES = {
     "f64_bucket_128": [0, 2, 3], "f64_bucket_192": [0, 2, 3], "f64_bucket_256": [0, 2, 3],
     "c128_bucket_128": [0, 3],
     "trail_f64": [0, 1], "trail_c128": [0, 1], "trail_sweep": [0, 1],
-    "panel_launch": [0, 1], "grad_blocked": [0, 1],
+    "panel_launch": [0], "grad_blocked": [0, 1],
 }
 BUCKETS = {"f64_bucket_128": (64, 128), "f64_bucket_192": (128, 192), "f64_bucket_256": (192, 256),
            "c128_bucket_128": (64, 128)}

@@ -28,4 +28,8 @@ size_t blocked_grad_workspace_bytes(int dtype, int64_t n);
 template <typename T>
 int blocked_grad(const T* A, int64_t n, void* ws, double* log_out, T* phase_out, T* G, hipStream_t s);
 
+// Synchronizes the stream, then reports (1) whether a panel's poll on the
+// current device gave up since the last call, and clears that report.
+int blocked_take_status(hipStream_t s, int* timed_out);
+
 }  // namespace pfh

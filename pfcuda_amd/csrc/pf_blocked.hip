@@ -75,6 +75,10 @@ constexpr unsigned long long BC_NONE_B = (0x7fffffffull << 32) | 0xffffffffull;
 
 __host__ __device__ constexpr unsigned long long bc_par(int e) { return (e & 1) ? BC_PAR : 0ull; }
 
+// Set by bp_finish / gj_out when a panel's poll gave up (the results of that
+// call are NaN); read and cleared by blocked_take_status.  One per device.
+__device__ int g_panel_timeout;
+
 template <typename T>
 struct BPState {
     int p, dead, counter, err;
@@ -452,26 +456,32 @@ constexpr bool panel_cand() { return !is_cplx<T>::value; }   // the winner's ent
 // word ever written decodes to indices in range, so nothing faults then.
 template <bool NF>
 __device__ __forceinline__ void bc_poll(const unsigned long long* rec_all, int e, int limit, int* err, int* s_w,
-                                        int* s_p, int* s_sp) {
+                                        int* s_p, int* s_sp, int* s_fail) {
     if (threadIdx.x < 64) {
         const int g = threadIdx.x;
         const int G = gridDim.x;
         const unsigned long long pb = bc_par(e);
         const unsigned long long* rec = rec_all + ((int64_t)(e % 3) * BC_MAXG + (g < G ? g : 0)) * 2;
         unsigned long long wa = BC_NONE_A, wb = BC_NONE_B;
+        int fail = 0;
         for (int reads = 0;; ++reads) {
             if (reads >= limit) {   // seconds by default: the grid is co-resident, so this is a failure
                 if (g == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                fail = 1;
                 break;
             }
             if (g < G) {
                 wa = __hip_atomic_load(&rec[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 wb = __hip_atomic_load(&rec[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
+            // another block that gave up has set err: its record may never come
+            const int failed = (g == 0) ? __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
             const bool ok = g >= G || ((((wa ^ pb) | (wb ^ pb)) & BC_PAR) == 0);
+            if (__shfl(failed, 0) != 0) { fail = 1; break; }
             if (__all(ok)) break;
             __builtin_amdgcn_s_sleep(1);
         }
+        if (g == 0) *s_fail = fail;
         if (g >= G) { wa = BC_NONE_A; wb = BC_NONE_B; }
         const double m = __longlong_as_double((long long)(wa & ~BC_PAR));
         const double best = (m == m) ? m : -1.0;
@@ -562,10 +572,10 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
     __shared__ __align__(16) T Bx[3][2 * NB];
     __shared__ double r_best[BC_WAVES];
     __shared__ int r_idx[BC_WAVES], r_t, r_slot;
-    __shared__ int s_w, s_p, s_sp;
+    __shared__ int s_w, s_p, s_sp, s_fail;
 
     BPState<T>* st = a.st;
-    if (st->dead) return;
+    if (st->dead || st->err) return;            // an earlier panel gave up: the output is NaN already
     const int64_t n = a.n;
     const int t = threadIdx.x;
     const int x = k0 + blockIdx.x * BC_T + t;   // own slot
@@ -603,7 +613,8 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         const bool has_next = i + 1 < np;       // row k+2 is this panel's next pivot row
         const int sa = Qpan[k + 1 - k0];
         const int sb = has_next ? Qpan[k + 2 - k0] : -1;
-        bc_poll<NF>(co.rec, e, a.spin, &st->err, &s_w, &s_p, &s_sp);
+        bc_poll<NF>(co.rec, e, a.spin, &st->err, &s_w, &s_p, &s_sp, &s_fail);
+        if (s_fail) return;                                // err is set: the result will be NaN
         const int wb = s_w;
         int p, sp;
         if (wb >= 0) { p = s_p; sp = s_sp; }
@@ -1042,7 +1053,12 @@ static bool trail_staged(int key, bool dflt) {
 template <typename T>
 __global__ void bp_finish(const BPState<T>* st, double* log_out, T* phase_out) {
     if (threadIdx.x != 0) return;
-    if (st->err) { *log_out = __builtin_nan(""); *phase_out = nan_of<T>(); return; }
+    if (st->err) {
+        atomicOr(&g_panel_timeout, 1);
+        *log_out = __builtin_nan("");
+        *phase_out = nan_of<T>();
+        return;
+    }
     if (st->dead) { *log_out = -INFINITY; *phase_out = zero<T>(); return; }
     *log_out = log(st->lm) + (double)st->le * 0.69314718055994530942;
     T ph;
@@ -1083,20 +1099,19 @@ int blocked_panel_width(int dtype) {
     return dtype == DT_C128 ? 16 : 32;
 }
 
-// The cooperative panel kernel runs when the device supports cooperative
-// launches and has at least BC_MAXG CUs (so that a panel's blocks, one per
-// CU, fit at once), unless PFH_BLOCKED_COOP=0, and while the panel's
-// columns fit BC_MAXG blocks of BC_T slots.
+// The panel kernel runs on a device with at least BC_MAXG CUs (so that a
+// panel's blocks, one per CU, are resident together), unless
+// PFH_BLOCKED_COOP=0, and while the panel's columns fit BC_MAXG blocks of
+// BC_T slots.
 static bool coop_enabled() {
     static DevCache cache;   // 1 enabled, 2 not
     return cache.get([] {
         const char* e = std::getenv("PFH_BLOCKED_COOP");
-        int dev = 0, ok = 0, cus = 0;
+        int dev = 0, cus = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&ok, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            ok = 0;
-        return (ok && cus >= BC_MAXG && !(e && e[0] == '0')) ? 1 : 2;
+            cus = 0;
+        return (cus >= BC_MAXG && !(e && e[0] == '0')) ? 1 : 2;
     }) == 1;
 }
 
@@ -1112,27 +1127,32 @@ static hipError_t launch_panel(const BPArgs<T>& a, const BPCoop<T>& co, int k0, 
             return err == hipSuccess ? 1 : 0;
         }))
         return err;
-    // Plain launch by default: G <= BC_MAXG blocks at one per CU (their LDS)
-    // on a device with more CUs than that are resident together once the
+    // A plain launch: G <= BC_MAXG blocks at one per CU (their LDS) on a
+    // device with more CUs than that are resident together once the
     // stream's earlier work has drained.  A block that cannot become
     // resident (the CUs held by other work for seconds) ends in the
-    // barrier's bounded spin, whose error makes the result NaN; nothing
-    // hangs.  PFH_PANEL_LAUNCH=coop asks the runtime for the co-residency
-    // guarantee instead (hipLaunchCooperativeKernel), at a measured cost of
-    // ~15 us per panel in a fresh process and ~100 us per panel in one that
-    // has captured and replayed graphs on several streams (N = 4096 f64:
-    // 12.6 ms plain, 13.5 / 20.5 ms cooperative).
-    const int tv = tune_value(PFH_TUNE_PANEL_LAUNCH);
-    const char* lm = std::getenv("PFH_PANEL_LAUNCH");
-    const bool coop = tv >= 0 ? tv == 1 : (lm && lm[0] == 'c');
-    if (!coop) {
-        hipLaunchKernelGGL((bp_panel<T, NB>), dim3(G), dim3(BC_T), smem, s, a, co, k0, np, epoch0);
-        return hipGetLastError();
+    // barrier's bounded spin, which sets the error flag: the result is NaN
+    // and pfh_stream_status reports PFH_ETIMEOUT; nothing hangs.
+    // (A cooperative launch, which asks the runtime for co-residency, was
+    // measured slower, 13.5-20.5 ms against 12.6 ms at N = 4096, and was
+    // removed.)
+    hipLaunchKernelGGL((bp_panel<T, NB>), dim3(G), dim3(BC_T), smem, s, a, co, k0, np, epoch0);
+    return hipGetLastError();
+}
+
+int blocked_take_status(hipStream_t s, int* timed_out) {
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return (int)e;
+    int v = 0;
+    e = hipMemcpyFromSymbol(&v, HIP_SYMBOL(g_panel_timeout), sizeof(int), 0, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return (int)e;
+    if (v) {
+        const int zero = 0;
+        e = hipMemcpyToSymbol(HIP_SYMBOL(g_panel_timeout), &zero, sizeof(int), 0, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return (int)e;
     }
-    BPArgs<T> a_ = a;
-    BPCoop<T> co_ = co;
-    void* args[] = {&a_, &co_, &k0, &np, &epoch0};
-    return hipLaunchCooperativeKernel((const void*)bp_panel<T, NB>, dim3(G), dim3(BC_T), args, (unsigned)smem, s);
+    *timed_out = v;
+    return 0;
 }
 
 size_t blocked_workspace_bytes(int dtype, int64_t n, bool mixed) {

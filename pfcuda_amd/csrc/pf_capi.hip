@@ -38,8 +38,8 @@ bool tune_value_ok(int key, int v) {
         case PFH_TUNE_TRAIL_F64:
         case PFH_TUNE_TRAIL_C128:
         case PFH_TUNE_TRAIL_SWEEP:
-        case PFH_TUNE_PANEL_LAUNCH:
         case PFH_TUNE_GRAD_BLOCKED: return v == 0 || v == 1;
+        case PFH_TUNE_PANEL_LAUNCH: return v == 0;   // the plain launch is the only form
         case PFH_TUNE_PANEL_SPIN: return v >= 0 && v <= (1 << 22);
     }
     return false;
@@ -201,7 +201,10 @@ RaggedLayout ragged_layout(int dt, int64_t count, int32_t max_n) {
     L.glb_grid = 0;
     L.slab_elems = 0;
     if (max_n > reg_max_n(dt)) {
-        L.glb_grid = glb_grid();
+        // one slab per workgroup of the global-slab kernel, which strides
+        // over the bucket: no more workgroups than matrices
+        const int64_t g = glb_grid();
+        L.glb_grid = (int)(count < g ? (count > 0 ? count : 1) : g);
         L.slab_elems = (int64_t)max_n * max_n;
         off = align_up(off + dtype_bytes(dt) * (size_t)L.slab_elems * L.glb_grid);
     }
@@ -405,6 +408,20 @@ int grad_dense(int dt, const T* A, int64_t batch, int32_t n, double* log_out, T*
     return 0;
 }
 
+// Device buffers of one host-pointer call, freed on every return path.
+struct DevBufs {
+    std::vector<void*> p;
+    ~DevBufs() { for (void* q : p) (void)hipFree(q); }
+    int alloc(size_t bytes, void** out) {
+        void* q = nullptr;
+        const hipError_t e = hipMalloc(&q, bytes ? bytes : 1);
+        if (e != hipSuccess) return (int)e;
+        p.push_back(q);
+        *out = q;
+        return 0;
+    }
+};
+
 }  // namespace
 
 int pfh::tune_value(int key) {
@@ -444,17 +461,67 @@ int pfh_slog_pfaffian_grad(int dtype, const void* A, int64_t batch, int32_t n, d
 }
 
 
+int pfh_pfaffian_host(int dtype, const void* h_A, int32_t n, void* h_out) {
+    const size_t es = dtype_bytes(dtype);
+    if (!es || n < 0 || !h_A || !h_out) return PFH_EINVAL;
+    if (n > 64) return PFH_ETOOBIG;
+    DevBufs b;
+    void *dA = nullptr, *dOut = nullptr;
+    int err = b.alloc(es * (size_t)n * n, &dA);
+    if (!err) err = b.alloc(es, &dOut);
+    if (err) return err;
+    hipError_t e = hipMemcpy(dA, h_A, es * (size_t)n * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return (int)e;
+    switch (dtype) {
+        case PFH_F32: err = pfaffian_dense<float>(dtype, dA, 1, n, dOut, nullptr); break;
+        case PFH_F64: err = pfaffian_dense<double>(dtype, dA, 1, n, dOut, nullptr); break;
+        case PFH_C64: err = pfaffian_dense<c64>(dtype, dA, 1, n, dOut, nullptr); break;
+        default: err = pfaffian_dense<c128>(dtype, dA, 1, n, dOut, nullptr); break;
+    }
+    if (err) return err;
+    e = hipMemcpy(h_out, dOut, es, hipMemcpyDeviceToHost);
+    return (int)e;
+}
+
+int pfh_slog_pfaffian_host(int dtype, const void* h_A, int32_t n, double* h_log, void* h_phase) {
+    const size_t es = dtype_bytes(dtype);
+    if (!es || n < 0 || !h_A || !h_log || !h_phase) return PFH_EINVAL;
+    DevBufs b;
+    void *dA = nullptr, *dLog = nullptr, *dPh = nullptr, *ws = nullptr;
+    const size_t wsb = slog_ws_bytes(dtype, 1, n, PFH_METHOD_AUTO);
+    int err = b.alloc(es * (size_t)n * n, &dA);
+    if (!err) err = b.alloc(sizeof(double), &dLog);
+    if (!err) err = b.alloc(es, &dPh);
+    if (!err && wsb) err = b.alloc(wsb, &ws);
+    if (err) return err;
+    hipError_t e = hipMemcpy(dA, h_A, es * (size_t)n * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return (int)e;
+    err = pfh_slog_pfaffian_method(dtype, dA, 1, n, PFH_METHOD_AUTO, (double*)dLog, dPh, ws, wsb, nullptr);
+    if (err) return err;
+    e = hipMemcpy(h_log, dLog, sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(h_phase, dPh, es, hipMemcpyDeviceToHost);
+    return (int)e;
+}
+
 const char* pfh_status_string(int status) {
     switch (status) {
         case PFH_OK: return "success";
         case PFH_EINVAL: return "invalid argument";
         case PFH_EWORKSPACE: return "workspace too small";
         case PFH_ETOOBIG: return "matrix size beyond this entry point";
+        case PFH_ETIMEOUT: return "a blocked panel gave up waiting for its blocks (that call's results are NaN)";
     }
     return hipGetErrorString((hipError_t)status);
 }
 
 int pfh_pfaffian_max_n(void) { return 64; }
+
+int pfh_stream_status(hipStream_t s) {
+    int timed_out = 0;
+    const int err = blocked_take_status(s, &timed_out);
+    if (err) return err;
+    return timed_out ? PFH_ETIMEOUT : PFH_OK;
+}
 
 int pfh_pfaffian_f32(const float* A, int64_t batch, int32_t n, float* out, hipStream_t s) {
     return pfaffian_dense<float>(PFH_F32, A, batch, n, out, s);

@@ -130,3 +130,39 @@ def test_autotune_cache_applies_for_its_architecture_only(tmp_path):
     finally:
         for name in autotune.CANDIDATES:
             autotune.set_tuning(name, -1)
+
+
+def test_panel_launch_key_takes_only_the_plain_form(lib):
+    # the cooperative launch form was removed: key 7 accepts 0 (and -1) only
+    lib.pfh_set_tuning.argtypes = [ctypes.c_int, ctypes.c_int]
+    assert lib.pfh_set_tuning(7, 1) == -1
+    assert lib.pfh_set_tuning(7, 0) == 0
+    assert lib.pfh_set_tuning(7, -1) == 0
+
+
+def test_host_forms_reject_bad_arguments(lib):
+    vp = ctypes.c_void_p
+    f = lib.pfh_pfaffian_host
+    f.argtypes = [ctypes.c_int, vp, ctypes.c_int32, vp]
+    buf = (ctypes.c_double * 4)()
+    assert f(1, None, 2, buf) == -1                  # null input
+    assert f(9, buf, 2, buf) == -1                   # unknown dtype
+    assert f(1, buf, -2, buf) == -1                  # negative n
+    assert f(1, buf, 66, buf) == -3                  # past the product-form limit
+    g = lib.pfh_slog_pfaffian_host
+    g.argtypes = [ctypes.c_int, vp, ctypes.c_int32, vp, vp]
+    assert g(1, buf, 4, None, buf) == -1             # null log output
+    lib.pfh_status_string.restype = ctypes.c_char_p
+    assert b"gave up" in lib.pfh_status_string(-4)
+
+
+def test_ragged_workspace_scales_with_count(lib):
+    # one global slab per matrix past the register kernels, not one per
+    # resident workgroup: a single n = 4096 f64 matrix needs about one
+    # n^2 slab
+    q = lib.pfh_slog_pfaffian_ragged_workspace_size
+    q.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_size_t)]
+    sz = ctypes.c_size_t(0)
+    assert q(1, 1, 4096, ctypes.byref(sz)) == 0
+    slab = 8 * 4096 * 4096
+    assert slab <= sz.value < 2 * slab

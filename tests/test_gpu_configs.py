@@ -1,0 +1,236 @@
+"""GPU parity at the benchmark configurations' full sizes, and the cases the
+smaller tests leave open, against the FP64 oracle.
+
+Tolerances (written in each test):
+  * slog at N = 4096 / 8192 (f64, c128): |dlog| <= 1e-9, sign equal (f64),
+    |dphase| <= 1e-9 (c128), as the single-matrix configurations state;
+  * ragged batches: the same bar per matrix;
+  * float32 / complex64 slog against the FP64 oracle on the same (rounded)
+    input: |dlog| and |dphase| <= 3e-8 n^2 + 1e-5 (single-precision
+    elimination, worst case: n/2 pivots, each with a relative error of up
+    to n u32, u32 = 6e-8), with the sign equal;
+  * derivatives of pfaffian: central differences with h = 1e-6 along skew
+    directions, relative 1e-6.
+The oracle runs threaded for the large matrices (bitwise the same as one
+thread)."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pf(cuda):
+    import pfcuda_amd
+    return pfcuda_amd
+
+
+_CACHE = {}
+
+
+def _c2_case(dtype):
+    """Config 2's input: one N = 4096 skew matrix, A = (B - B^T)/2 from a
+    seed (complex: re, im ~ N(0,1)/sqrt2), and its oracle (log, sign/phase)."""
+    key = ("c2", dtype)
+    if key not in _CACHE:
+        rng = np.random.default_rng(4096)
+        A = oracle.skew_gaussian(rng, 4096, complex_=dtype.is_complex)
+        _CACHE[key] = (A, oracle.slog(A))
+    return _CACHE[key]
+
+
+@pytest.mark.parametrize("method", ["blocked", "unblocked"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
+def test_c2_n4096_vs_oracle(pf, cuda, dtype, method):
+    A, (l0, p0) = _c2_case(dtype)
+    m = {"blocked": pf.PFH_METHOD_BLOCKED, "unblocked": pf.PFH_METHOD_UNBLOCKED}[method]
+    la, ph = pf.slog_pfaffian(torch.from_numpy(A).to(cuda), m)
+    pf.check_status()
+    la, ph = float(la), complex(ph.item())
+    assert abs(la - l0) <= 1e-9, (la, l0)
+    if dtype.is_complex:
+        assert abs(ph - complex(p0)) <= 1e-9, (ph, p0)
+    else:
+        assert ph.real == p0, (ph, p0)
+
+
+def _c3_case(cuda):
+    """Config 3's input: A = Q blockdiag([0, s_i; -s_i, 0]) Q^T at N = 8192,
+    Q random orthogonal (QR with the sign fix), s_i log-uniform in
+    [1e-3, 1e3], skew-symmetrized; with the closed-form sign det(Q) and the
+    oracle's (log, sign)."""
+    if "c3" not in _CACHE:
+        n = 8192
+        g = torch.Generator(device=cuda)
+        g.manual_seed(8192)
+        Q, R = torch.linalg.qr(torch.randn(n, n, generator=g, device=cuda, dtype=torch.float64))
+        Q = Q * torch.sign(torch.diagonal(R))
+        del R
+        s = 10.0 ** (torch.rand(n // 2, generator=g, device=cuda, dtype=torch.float64) * 6 - 3)
+        D = torch.zeros(n, n, dtype=torch.float64, device=cuda)
+        i = torch.arange(0, n, 2, device=cuda)
+        D[i, i + 1] = s
+        D[i + 1, i] = -s
+        M = Q @ D @ Q.T
+        del D
+        A = ((M - M.T) / 2).contiguous()
+        del M
+        sign_closed = float(torch.sign(torch.linalg.det(Q)).item())
+        del Q
+        Ah = A.cpu().numpy()
+        _CACHE["c3"] = (A, sign_closed, oracle.slog(Ah))
+    return _CACHE["c3"]
+
+
+@pytest.mark.parametrize("method", ["blocked", "auto"])
+def test_c3_n8192_vs_oracle(pf, cuda, method):
+    A, sign_closed, (l0, s0) = _c3_case(cuda)
+    m = {"blocked": pf.PFH_METHOD_BLOCKED, "auto": pf.PFH_METHOD_AUTO}[method]
+    la, s = pf.slog_pfaffian(A, m)
+    pf.check_status()
+    assert abs(float(la) - l0) <= 1e-9, (float(la), l0)
+    assert float(s) == s0 == sign_closed
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
+def test_ragged_queue_reclaims_vs_oracle(pf, cuda, dtype):
+    # More matrices per bucket than the bucket kernel has resident
+    # workgroups (768 for the 192 < n <= 256 bucket's batched blocked
+    # kernel), so workgroups claim a second and third matrix from the
+    # bucket's queue, and the column-per-lane kernel's stride loop makes
+    # further passes over the n <= 64 buckets.
+    cplx = dtype.is_complex
+    rng = np.random.default_rng(2500)
+    sizes = ([int(x) for x in 2 * rng.integers(97, 129, size=2500)]        # 194 .. 256
+             + [int(x) for x in 2 * rng.integers(33, 97, size=600)]        # 66 .. 192
+             + [int(x) for x in 2 * rng.integers(8, 33, size=6000)])       # 16 .. 64
+    rng.shuffle(sizes)
+    mats = [oracle.skew_gaussian(rng, n, complex_=cplx) for n in sizes]
+    data = np.concatenate([m.reshape(-1) for m in mats])
+    offs = np.cumsum([0] + [n * n for n in sizes[:-1]]).astype(np.int64)
+    del mats
+    l0, p0 = oracle.slog_ragged(data, offs, np.array(sizes, dtype=np.int32), threads=oracle.default_threads())
+    gl, gp = pf.slog_pfaffian_ragged(torch.from_numpy(data).to(cuda), offs, sizes, max_n=256)
+    gl, gp = gl.cpu().numpy(), gp.cpu().numpy()
+    dl = np.abs(gl - l0)
+    assert dl.max() <= 1e-9, (int(dl.argmax()), sizes[int(dl.argmax())], dl.max())
+    if cplx:
+        assert np.abs(gp - p0).max() <= 1e-9
+    else:
+        assert np.array_equal(gp, p0)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
+def test_pfaffian_jvp_and_backward_vs_central_differences(pf, cuda, dtype):
+    # pfaffian's derivative rules (reference pfcuda/cuda_api.py:78-90):
+    # forward mode d Pf = 1/2 Pf tr(A^-1 dA), reverse mode A_bar =
+    # g conj(1/2 Pf A^-T), both against a central difference of the oracle
+    # along a random skew direction E (h = 1e-6, relative 1e-6)
+    cplx = dtype.is_complex
+    rng = np.random.default_rng(90)
+    h = 1e-6
+    for n in (6, 16, 32, 64):
+        A = oracle.skew_gaussian(rng, n, complex_=cplx)
+        E = oracle.skew_gaussian(rng, n, complex_=cplx)
+        f = (lambda M: oracle.pf_c128(M)[0]) if cplx else (lambda M: oracle.pf_f64(M)[0])
+        fd = (f(A + h * E) - f(A - h * E)) / (2 * h)
+        At = torch.from_numpy(A).to(cuda)
+        Et = torch.from_numpy(E).to(cuda)
+        _, tangent = torch.func.jvp(pf.pfaffian, (At,), (Et,))
+        tangent = complex(tangent.item())
+        assert abs(tangent - fd) <= 1e-6 * abs(fd), (n, tangent, fd)
+        Ar = At.clone().requires_grad_(True)
+        p = pf.pfaffian(Ar)
+        (G,) = torch.autograd.grad(p, Ar, grad_outputs=torch.ones_like(p))
+        directional = complex((G.conj() * Et).sum().item())
+        assert abs(directional - fd) <= 1e-6 * abs(fd), (n, directional, fd)
+
+
+def _single_tol(n):
+    return 3e-8 * n * n + 1e-5
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.complex64])
+def test_slog_single_precision_dense_vs_fp64_oracle(pf, cuda, dtype):
+    # the register kernels serve float32 up to n = 256 and complex64 up to
+    # 192; past that the dense call takes the blocked path
+    cplx = dtype.is_complex
+    rng = np.random.default_rng(32)
+    for n in (34, 64, 100, 128, 192, 256, 300, 600):
+        A = oracle.skew_gaussian(rng, n, complex_=cplx, batch=2)
+        A32 = torch.from_numpy(A).to(dtype)
+        Ar = A32.to(torch.complex128 if cplx else torch.float64).numpy()   # the rounded input, exactly
+        gl, gp = pf.slog_pfaffian(A32.to(cuda))
+        for a, l1, p1 in zip(Ar, gl.cpu().numpy(), gp.cpu().numpy()):
+            l0, p0 = oracle.slog(a)
+            assert abs(l1 - l0) <= _single_tol(n), (n, l1, l0)
+            if cplx:
+                assert abs(complex(p1) - p0) <= _single_tol(n), (n, p1, p0)
+            else:
+                assert float(p1) == p0, (n, p1, p0)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.complex64])
+def test_slog_single_precision_blocked_and_ragged(pf, cuda, dtype):
+    cplx = dtype.is_complex
+    rng = np.random.default_rng(33)
+    # blocked (n past the register kernels) and unblocked forms
+    A = oracle.skew_gaussian(rng, 520, complex_=cplx)
+    A32 = torch.from_numpy(A).to(dtype)
+    Ar = A32.to(torch.complex128 if cplx else torch.float64).numpy()
+    l0, p0 = oracle.slog(Ar)
+    for m in (pf.PFH_METHOD_BLOCKED, pf.PFH_METHOD_UNBLOCKED):
+        gl, gp = pf.slog_pfaffian(A32.to(cuda), m)
+        assert abs(float(gl) - l0) <= _single_tol(520), (m, float(gl), l0)
+        if cplx:
+            assert abs(complex(gp.item()) - p0) <= _single_tol(520)
+        else:
+            assert float(gp) == p0
+    # ragged: every bucket, and one size past the register kernels
+    sizes = [int(x) for x in 2 * rng.integers(8, 129, size=200)] + [300, 2, 0, 7]
+    mats = [torch.from_numpy(oracle.skew_gaussian(rng, n, complex_=cplx)).to(dtype) for n in sizes]
+    data, offs, sz = pf.pack_ragged(mats)
+    gl, gp = pf.slog_pfaffian_ragged(data.to(cuda), offs, sz)
+    gl, gp = gl.cpu().numpy(), gp.cpu().numpy()
+    for i, m in enumerate(mats):
+        l0, p0 = oracle.slog(m.to(torch.complex128 if cplx else torch.float64).numpy())
+        if math.isinf(l0):
+            assert math.isinf(gl[i]) and gl[i] < 0 and gp[i] == 0, (i, sizes[i])
+            continue
+        assert abs(gl[i] - l0) <= _single_tol(max(sizes[i], 1)), (i, sizes[i], gl[i], l0)
+        if cplx:
+            assert abs(complex(gp[i]) - p0) <= _single_tol(sizes[i])
+        else:
+            assert float(gp[i]) == p0, (i, sizes[i])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128, np.float32, np.complex64])
+def test_host_pointer_forms_vs_oracle(pf, cuda, dtype):
+    # the synchronous host-pointer entry points (reference src/pfaffian.cu:4-40,
+    # src/slog_pfaffian.cu:4-32) on host matrices
+    cplx = np.iscomplexobj(np.zeros(1, dtype=dtype))
+    lo = dtype in (np.float32, np.complex64)
+    rng = np.random.default_rng(41)
+    for n in (0, 2, 4, 7, 16, 32, 64):
+        A = oracle.skew_gaussian(rng, n, complex_=cplx).astype(dtype) if n else np.zeros((0, 0), dtype)
+        got = complex(pf.pfaffian_host(A))
+        Ar = A.astype(np.complex128 if cplx else np.float64)
+        want = (oracle.pf_c128(Ar)[0] if cplx else oracle.pf_f64(Ar)[0]) if n else 0.0
+        if n == 0 or n & 1:
+            assert got == 0
+        else:
+            assert abs(got - want) <= (1e-4 if lo else 1e-12) * abs(want), (n, got, want)
+    for n in (40, 300, 600):
+        A = oracle.skew_gaussian(rng, n, complex_=cplx).astype(dtype)
+        la, ph = pf.slog_pfaffian_host(A)
+        l0, p0 = oracle.slog(A.astype(np.complex128 if cplx else np.float64))
+        tol = _single_tol(n) if lo else 1e-9
+        assert abs(la - l0) <= tol, (n, la, l0)
+        assert abs(complex(ph) - complex(p0)) <= tol
+    la, ph = pf.slog_pfaffian_host(np.zeros((7, 7), dtype=dtype))
+    assert la == -math.inf and ph == 0

@@ -184,9 +184,9 @@ def test_ragged_above_256_and_beyond_max_n(pf, cuda, dtype):
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
 def test_panel_poll_limit_gives_nan_then_recovers(pf, cuda, dtype):
-    # The cooperative panel's wait for the other blocks' candidate records is
-    # bounded: past its limit it sets the panel's error flag, the output is
-    # NaN, and nothing hangs.  The limit is forced to zero from the host
+    # The panel's wait for the other blocks' candidate records is bounded:
+    # past its limit it sets the panel's error flag, the output is NaN,
+    # pfh_stream_status reports PFH_ETIMEOUT, and nothing hangs.  The limit is forced to zero from the host
     # (polls give up before reading), for the solve and for the gradient's
     # block sweep; the next calls, at the default limit, are exact again.
     from pfcuda_amd import _lib
@@ -196,13 +196,20 @@ def test_panel_poll_limit_gives_nan_then_recovers(pf, cuda, dtype):
     At = torch.from_numpy(A).to(cuda)
     l0, p0 = oracle.slog(A)
     spin, gb = _lib.TUNE_KEYS["panel_spin"], _lib.TUNE_KEYS["grad_blocked"]
+    pf.check_status()                          # nothing pending from earlier tests
     try:
         _lib.check(_lib.lib.pfh_set_tuning(gb, 1), "grad_blocked")
         _lib.check(_lib.lib.pfh_set_tuning(spin, 0), "panel_spin")
         la, ph = pf.slog_pfaffian(At, pf.PFH_METHOD_BLOCKED)
         assert math.isnan(float(la)) and torch.isnan(torch.as_tensor(ph)).all()
+        # the status call reports the timeout once, then it is cleared
+        with pytest.raises(_lib.PfaffianLibError, match="status -4"):
+            pf.check_status()
+        pf.check_status()
         G = slog_grad(At)
         assert torch.isnan(G).all()
+        with pytest.raises(_lib.PfaffianLibError, match="status -4"):
+            pf.check_status()
     finally:
         _lib.lib.pfh_set_tuning(spin, -1)
     try:
@@ -211,6 +218,7 @@ def test_panel_poll_limit_gives_nan_then_recovers(pf, cuda, dtype):
         G = slog_grad(At).cpu().numpy()
         Gi = 0.5 * np.linalg.inv(A).T
         assert np.abs(G - Gi).max() <= 1e-9 * np.abs(Gi).max()
+        pf.check_status()                      # exact calls report nothing
     finally:
         _lib.lib.pfh_set_tuning(gb, -1)
 
@@ -532,25 +540,6 @@ def test_trailing_forms_bitwise_equal(pf, cuda, monkeypatch, dtype):
     assert np.array_equal(res["1"][2], res["0"][2])
     l0, p0 = oracle.slog(A)
     assert abs(res["1"][0] - l0) < 1e-9 and abs(res["1"][1] - p0) < 1e-9
-
-
-@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
-def test_panel_launch_forms_bitwise_equal(pf, cuda, monkeypatch, dtype):
-    # The panel kernel launched plainly (the default) and through the
-    # runtime's cooperative launch (PFH_PANEL_LAUNCH=coop) is the same code
-    # on the same grid: results agree bit for bit.
-    from pfcuda_amd.adjoint import slog_grad
-    rng = np.random.default_rng(62)
-    A = oracle.skew_gaussian(rng, 700, complex_=dtype.is_complex)
-    At = torch.from_numpy(A).to(cuda)
-    res = {}
-    for form in ("plain", "coop"):
-        monkeypatch.setenv("PFH_PANEL_LAUNCH", form)
-        la, s = pf.slog_pfaffian(At, method=pf.PFH_METHOD_BLOCKED)
-        G = slog_grad(At)
-        res[form] = (float(la), complex(s.item()), G.cpu().numpy())
-    assert res["plain"][0] == res["coop"][0] and res["plain"][1] == res["coop"][1]
-    assert np.array_equal(res["plain"][2], res["coop"][2])
 
 
 def test_grad_n2048_residual(pf, cuda):

@@ -92,6 +92,64 @@ def test_against_mpmath_fixture(kind):
         assert abs(py_la - la) < 1e-11 and abs(py_ph - ph) < 1e-10
 
 
+@pytest.mark.parametrize("kind", ["real", "complex"])
+def test_long_double_runs_against_mpmath_fixture(kind):
+    # pf_f64_ld / pf_c128_ld against the committed 50-digit values: tighter
+    # than the double runs are held to
+    g = np.load(os.path.join(GOLD, "mp_slog.npz"))
+    for A, la, ph in zip(unpack(g[f"{kind}_data"], g[f"{kind}_sizes"]), g[f"{kind}_log"],
+                         g[f"{kind}_phase"]):
+        if kind == "complex":
+            _, got_la, got_ph, _ = oracle.pf_c128_ld(A)
+        else:
+            _, got_la, got_ph, _ = oracle.pf_f64_ld(A)
+        assert abs(got_la - la) < 2e-14 * max(1.0, abs(la))
+        assert abs(got_ph - ph) < 1e-14
+
+
+def test_long_double_complex_run_is_closer_to_mpmath():
+    rng = np.random.default_rng(12)
+    A = oracle.skew_gaussian(rng, 40, complex_=True)
+    mla, mph, _ = oracle.pfaffian_mp(A, 60)
+    _, la64, ph64, _ = oracle.pf_c128(A)
+    _, la80, ph80, _ = oracle.pf_c128_ld(A)
+    assert abs(la80 - mla) <= abs(la64 - mla) + 1e-15
+    assert abs(la80 - mla) < 1e-14 and abs(ph80 - mph) < 1e-14
+
+
+def test_long_double_complex_brute_force_gaussian_integers():
+    # Gaussian-integer skew matrices: the expansion over perfect matchings is
+    # exact, and the extended run must give it to ~1e-17 relative; a dropped
+    # term, a sign or a conjugate in the complex path fails by O(1)
+    rng = np.random.default_rng(13)
+    for n in (4, 6, 8, 10):
+        for _ in range(3):
+            B = rng.integers(-5, 6, size=(n, n)) + 1j * rng.integers(-5, 6, size=(n, n))
+            A = B - B.T
+            exact = complex(oracle.pfaffian_brute(A.tolist()))
+            prod, la, ph, _ = oracle.pf_c128_ld(A)
+            if exact == 0:
+                assert abs(prod) < 1e-9
+                continue
+            assert abs(prod - exact) <= 1e-15 * abs(exact)
+            assert abs(la - math.log(abs(exact))) < 1e-15 * max(1.0, abs(math.log(abs(exact))))
+            assert abs(ph - exact / abs(exact)) < 1e-15
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_threaded_oracle_is_bitwise_equal(cplx):
+    # the threaded form splits each pair step's column loops; every column
+    # keeps its operation order, so the results are the same bits
+    rng = np.random.default_rng(14)
+    for n in (6, 130, 512, 700):
+        A = oracle.skew_gaussian(rng, n, complex_=cplx)
+        want = oracle.pf_c128(A) if cplx else oracle.pf_f64(A)
+        for threads in (1, 3, 8):
+            got = oracle.pf_c128_mt(A, threads) if cplx else oracle.pf_f64_mt(A, threads)
+            # the product form overflows to NaN/inf past n ~ 600: equal as NaN
+            assert np.array_equal(np.array(got, dtype=complex), np.array(want, dtype=complex), equal_nan=True)
+
+
 def test_long_double_run_is_closer_to_mpmath():
     rng = np.random.default_rng(11)
     A = oracle.skew_gaussian(rng, 40)
