@@ -108,6 +108,26 @@ int pfh_slog_pfaffian_ragged(int dtype, const void* data, const int64_t* offsets
                              int64_t count, int32_t max_n, double* log_out, void* phase_out,
                              void* workspace, size_t workspace_bytes, hipStream_t stream);
 
+/* pfh_slog_pfaffian_ragged without any allocation: no counterpart in the
+ * reference; for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  The
+ * workspace is a fixed device array inside the library,
+ * PFH_NOALLOC_SCRATCH_BYTES (256 MiB), which every device the library's code
+ * is loaded on holds until the process ends.  Limits: max_n <=
+ * PFH_NOALLOC_MAX_N (256) and count <= PFH_NOALLOC_MAX_COUNT (2^22), else
+ * PFH_ETOOBIG.  The batched blocked kernel (f64 / c128, 64 < n <= 256) runs
+ * with as many workgroups as slab pairs fit in the array (fewer than
+ * pfh_slog_pfaffian_ragged may use, same results).  Every call on a device
+ * shares the one array, and each call clears what must start at zero:
+ * calls that overlap on one device (other streams, other host threads)
+ * race on it and must not be made. */
+#define PFH_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+#define PFH_NOALLOC_MAX_N 256
+#define PFH_NOALLOC_MAX_COUNT ((int64_t)1 << 22)
+int pfh_slog_pfaffian_ragged_noalloc(int dtype, const void* data, const int64_t* offsets, const int32_t* sizes,
+                                     int64_t count, int32_t max_n, double* log_out, void* phase_out,
+                                     hipStream_t stream);
+
 /* Workspace bytes for pfh_slog_pfaffian_grad. */
 int pfh_slog_pfaffian_grad_workspace_size(int dtype, int64_t batch, int32_t n, size_t* bytes);
 

@@ -58,6 +58,7 @@ def _load():
     lib.pfh_pfaffian_host.argtypes = [ctypes.c_int, _vp, _i32, _vp]
     lib.pfh_slog_pfaffian_host.argtypes = [ctypes.c_int, _vp, _i32, _vp, _vp]
     lib.pfh_stream_status.argtypes = [_vp]
+    lib.pfh_slog_pfaffian_ragged_noalloc.argtypes = [ctypes.c_int, _vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp]
     lib.pfh_set_tuning.argtypes = [ctypes.c_int, ctypes.c_int]
     lib.pfh_get_tuning.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     return lib

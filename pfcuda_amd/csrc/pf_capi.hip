@@ -177,6 +177,7 @@ struct RaggedLayout {
     size_t hist, cursor, order, begin, cnt, edges, queue, spill, bat, slabs, total;
     int glb_grid;
     int64_t slab_elems;
+    int64_t bat_elems;   // doubles of slab pairs for the batched blocked kernel
 };
 
 int glb_grid() { return 2 * device_cu_count(); }
@@ -195,8 +196,11 @@ RaggedLayout ragged_layout(int dt, int64_t count, int32_t max_n) {
     if (dt == PFH_F64 && max_n > lds_max_n())   // spill slabs of the LDS kernel's 192 < n <= 256 bucket
         off = align_up(off + sizeof(double) * (size_t)lds_spill_elems() * lds_spill_slabs());
     L.bat = off;
-    if ((dt == PFH_F64 || dt == PFH_C128) && max_n > 64)   // slab pairs of the batched blocked kernel
-        off = align_up(off + sizeof(double) * (size_t)bat_ws_elems(dt, max_n, count));
+    L.bat_elems = 0;
+    if ((dt == PFH_F64 || dt == PFH_C128) && max_n > 64) {   // slab pairs of the batched blocked kernel
+        L.bat_elems = bat_ws_elems(dt, max_n, count);
+        off = align_up(off + sizeof(double) * (size_t)L.bat_elems);
+    }
     L.slabs = off;
     L.glb_grid = 0;
     L.slab_elems = 0;
@@ -259,7 +263,7 @@ int ragged_typed(int dt, const void* data, const int64_t* offsets, const int32_t
         }
         if ((dt == PFH_F64 || dt == PFH_C128) && max_n > 64) {
             d.bslab = (double*)(ws + L.bat);
-            d.bslab_elems = bat_ws_elems(dt, max_n, count);
+            d.bslab_elems = L.bat_elems;
             d.queue = queue + b;            // used by the batched blocked kernel
         }
         if (b >= nb) {
@@ -421,6 +425,58 @@ struct DevBufs {
         return 0;
     }
 };
+
+// ---------------------------------------------------------------------------
+// Allocation-free ragged slog: the workspace is a fixed device array of the
+// library's own, so a call makes no hipMalloc (for code under graph capture
+// or loops that must not allocate).  Each device holds its own copy from the
+// time the library's code is loaded on it until the process ends.  Host
+// code never passes `scratch` itself to a kernel (in host code the name is a
+// stand-in, not a device address): the address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = (size_t)256 << 20;   // PFH_NOALLOC_SCRATCH_BYTES
+constexpr int32_t kNoallocMaxN = 256;                  // PFH_NOALLOC_MAX_N
+constexpr int64_t kNoallocMaxCount = (int64_t)1 << 22; // PFH_NOALLOC_MAX_COUNT
+static_assert(kScratchBytes == ((size_t)256 << 20), "the header states 256 MiB");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::atomic<void*> g_scratch_ptr[kMaxDevices];
+
+int scratch_address(void** out) {
+    const int dev = current_device();
+    void* p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+    if (!p) {
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        if (!p) return (int)hipErrorInvalidSymbol;
+        g_scratch_ptr[dev].store(p, std::memory_order_release);
+    }
+    *out = p;
+    return 0;
+}
+
+// The ragged layout with the batched blocked kernel's slab pairs cut to the
+// bytes left in `cap` (its grid shrinks to the pairs it is given): false when
+// even the fixed part, or one slab pair of the widest bucket, does not fit.
+bool ragged_layout_capped(int dt, int64_t count, int32_t max_n, size_t cap, RaggedLayout* out) {
+    RaggedLayout L = ragged_layout(dt, count, max_n);
+    if (L.glb_grid != 0) return false;                 // sizes past the register kernels: n^2 slabs
+    if (L.bat_elems > 0) {
+        if (L.bat > cap) return false;
+        const int64_t room = (int64_t)((cap - L.bat) / sizeof(double));
+        const int64_t np = max_n > 192 ? 256 : (max_n > 128 ? 192 : 128);   // the widest bucket's slabs
+        const int64_t need = 2LL * np * np * (dt == PFH_C128 ? 2 : 1);      // one pair of them
+        if (room < need) return false;
+        if (L.bat_elems > room) L.bat_elems = room;
+        L.slabs = align_up(L.bat + sizeof(double) * (size_t)L.bat_elems);
+        L.total = L.slabs;
+    }
+    if (L.total > cap) return false;
+    *out = L;
+    return true;
+}
 
 }  // namespace
 
@@ -586,6 +642,27 @@ int pfh_slog_pfaffian_ragged(int dtype, const void* data, const int64_t* offsets
     const RaggedLayout L = ragged_layout(dtype, count, max_n);
     void* ws = nullptr;
     int err = resolve_ws(ws_in, ws_bytes, L.total, &ws);
+    if (err) return err;
+    switch (dtype) {
+        case PFH_F32: return ragged_typed<float>(dtype, data, offsets, sizes, count, max_n, log_out, phase_out, (char*)ws, L, s);
+        case PFH_F64: return ragged_typed<double>(dtype, data, offsets, sizes, count, max_n, log_out, phase_out, (char*)ws, L, s);
+        case PFH_C64: return ragged_typed<c64>(dtype, data, offsets, sizes, count, max_n, log_out, phase_out, (char*)ws, L, s);
+        case PFH_C128: return ragged_typed<c128>(dtype, data, offsets, sizes, count, max_n, log_out, phase_out, (char*)ws, L, s);
+    }
+    return PFH_EINVAL;
+}
+
+int pfh_slog_pfaffian_ragged_noalloc(int dtype, const void* data, const int64_t* offsets, const int32_t* sizes,
+                                     int64_t count, int32_t max_n, double* log_out, void* phase_out,
+                                     hipStream_t s) {
+    if (dtype_bytes(dtype) == 0 || count < 0 || max_n < 0) return PFH_EINVAL;
+    if (count == 0) return 0;
+    if (!data || !offsets || !sizes || !log_out || !phase_out) return PFH_EINVAL;
+    if (max_n > kNoallocMaxN || count > kNoallocMaxCount) return PFH_ETOOBIG;
+    RaggedLayout L{};
+    if (!ragged_layout_capped(dtype, count, max_n, kScratchBytes, &L)) return PFH_ETOOBIG;
+    void* ws = nullptr;
+    const int err = scratch_address(&ws);
     if (err) return err;
     switch (dtype) {
         case PFH_F32: return ragged_typed<float>(dtype, data, offsets, sizes, count, max_n, log_out, phase_out, (char*)ws, L, s);
