@@ -181,7 +181,9 @@ int pfh_pfaffian_cpu_f64(double* A, int32_t n, double* out);
 #define PFH_TUNE_PANEL_SPIN 9       /* blocked panel: reads of the candidate records a poll makes before the
                                        panel sets its error flag (output NaN), 1 .. 4194304 (the default);
                                        0 sets it at once.  For tests of that path, not a tuning choice */
-#define PFH_TUNE_NKEYS 10
+#define PFH_TUNE_PANEL_XCD 10      /* blocked panel exchange: 1 through one XCD's L2 when the panel's blocks
+                                       fit one XCD, 0 device-wide; built-in: 1 for complex, 0 for real types */
+#define PFH_TUNE_NKEYS 11
 
 /* Sets a tuning key (value -1: the built-in choice); PFH_EINVAL for an
  * unknown key or a value outside the key's list.  Takes effect for calls

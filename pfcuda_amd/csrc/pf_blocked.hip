@@ -97,9 +97,10 @@ struct BPArgs {
     double* part_v;  // per-block argmax partials
     int* part_i;
     BPState<T>* st;
+    int* claim;      // per panel launch: the XCD-local panel's rank counter (n / (2 nb) + 2 of them)
     int64_t n;
     int nb;
-    int spin;        // the cooperative panel's poll: reads of the records before its error flag
+    int spin;        // the panel's poll: reads of the records before its error flag
 };
 
 // Element (r, c) of the stale matrix seen through perm, from the upper triangle.
@@ -161,6 +162,9 @@ __global__ void bp_init(BPArgs<T> a, unsigned long long* rec) {
     const int64_t n = a.n;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         a.perm[i] = (int)i;
+    const int64_t nclaim = n / (2 * a.nb) + 2;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nclaim; i += (int64_t)gridDim.x * blockDim.x)
+        a.claim[i] = 0;
     // candidate records: "no candidate", with the parity opposite to that of
     // the first epoch each buffer serves (epochs count from 1: buffer 1
     // serves 1, 4, 7, ...; buffer 2 serves 2, 5, ...; buffer 0 serves 3, 6, ...)
@@ -427,25 +431,90 @@ struct BPCoop {
 };
 
 // Accesses to the data the panel's blocks exchange within a launch (panel
-// vectors).  NF (real types): agent-scope relaxed atomics,
-// coherent across the XCDs' L2s on their own, so that a record needs no L2
-// write-back / invalidate, only a wait for the block's stores before it
-// (N = 4096 f64 13.5 -> 12.7 ms, N = 8192 33.6 -> 30.9 ms, results
-// bit-identical).  Complex types keep plain accesses and agent-scope
-// fences: split into two atomics per value they were slower (c128 N = 4096
-// 19.4 -> 20.5 ms).
-template <bool NF, typename V>
+// vectors, candidate rows), by exchange mode XM:
+//   XM_AGENT (real types, blocks anywhere): agent-scope relaxed atomics,
+//     coherent across the XCDs' L2s on their own, so that a record needs no
+//     L2 write-back / invalidate, only a wait for the block's stores before
+//     it (N = 4096 f64 13.5 -> 12.7 ms, N = 8192 33.6 -> 30.9 ms, results
+//     bit-identical);
+//   XM_FENCE (complex types, blocks anywhere): plain accesses and
+//     agent-scope fences (as two atomics per value they were slower, c128
+//     N = 4096 19.4 -> 20.5 ms);
+//   XM_XCD (every type, all blocks on one XCD): stores at workgroup scope
+//     (written through to the XCD's L2) and loads at system scope (they
+//     miss the CU's L1 and are served by that L2), so an exchange never
+//     leaves the XCD.  A flag round trip between two workgroups of one XCD
+//     measured 420-470 ns this way against 810-1240 ns with agent-scope
+//     accesses (which go past the L2); across two XCDs this form is not
+//     coherent, which is why the panel checks its blocks' XCD.
+constexpr int XM_AGENT = 0, XM_FENCE = 1, XM_XCD = 2;
+
+template <int XM, typename V>
 __device__ __forceinline__ void xb_st(V* p, V v) {
-    if constexpr (NF) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else *p = v;
+    if constexpr (XM == XM_XCD) {
+        if constexpr (sizeof(V) == 16) {
+            unsigned long long w[2];
+            __builtin_memcpy(w, &v, 16);
+            unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
+            __hip_atomic_store(q, w[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(q + 1, w[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else if constexpr (sizeof(V) == 8) {
+            unsigned long long w;
+            __builtin_memcpy(&w, &v, 8);
+            __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), w, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+            static_assert(sizeof(V) == 4, "exchanged values are 4, 8 or 16 bytes");
+            unsigned w;
+            __builtin_memcpy(&w, &v, 4);
+            __hip_atomic_store(reinterpret_cast<unsigned*>(p), w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    } else if constexpr (XM == XM_AGENT) {
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        *p = v;
+    }
 }
-template <bool NF, typename V>
+template <int XM, typename V>
 __device__ __forceinline__ V xb_ld(const V* p) {
-    if constexpr (NF) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else return *p;
+    if constexpr (XM == XM_XCD) {
+        V v;
+        if constexpr (sizeof(V) == 16) {
+            const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
+            unsigned long long w[2];
+            w[0] = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            w[1] = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __builtin_memcpy(&v, w, 16);
+        } else if constexpr (sizeof(V) == 8) {
+            const unsigned long long w = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
+                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __builtin_memcpy(&v, &w, 8);
+        } else {
+            const unsigned w = __hip_atomic_load(reinterpret_cast<const unsigned*>(p), __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_SYSTEM);
+            __builtin_memcpy(&v, &w, 4);
+        }
+        return v;
+    } else if constexpr (XM == XM_AGENT) {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        return *p;
+    }
 }
-template <typename T>
-constexpr bool panel_nf() { return !is_cplx<T>::value; }
+// record words: the same scopes as the data for the XCD-local form, agent
+// scope otherwise
+template <int XM>
+__device__ __forceinline__ void rec_st(unsigned long long* p, unsigned long long v) {
+    if constexpr (XM == XM_XCD) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <int XM>
+__device__ __forceinline__ unsigned long long rec_ld(const unsigned long long* p) {
+    if constexpr (XM == XM_XCD) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    else return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T, bool LOCAL>
+constexpr int panel_xm() { return LOCAL ? XM_XCD : (is_cplx<T>::value ? XM_FENCE : XM_AGENT); }
 template <typename T>
 constexpr bool panel_cand() { return !is_cplx<T>::value; }   // the winner's entries through a candidate row
 
@@ -454,12 +523,11 @@ constexpr bool panel_cand() { return !is_cplx<T>::value; }   // the winner's ent
 // candidate, with its logical index and slot.  A record that does not show
 // up within the spin limit sets err (the output becomes NaN); every record
 // word ever written decodes to indices in range, so nothing faults then.
-template <bool NF>
-__device__ __forceinline__ void bc_poll(const unsigned long long* rec_all, int e, int limit, int* err, int* s_w,
-                                        int* s_p, int* s_sp, int* s_fail) {
+template <int XM>
+__device__ __forceinline__ void bc_poll(const unsigned long long* rec_all, int e, int G, int limit, int* err,
+                                        int* s_w, int* s_p, int* s_sp, int* s_fail) {
     if (threadIdx.x < 64) {
         const int g = threadIdx.x;
-        const int G = gridDim.x;
         const unsigned long long pb = bc_par(e);
         const unsigned long long* rec = rec_all + ((int64_t)(e % 3) * BC_MAXG + (g < G ? g : 0)) * 2;
         unsigned long long wa = BC_NONE_A, wb = BC_NONE_B;
@@ -471,11 +539,14 @@ __device__ __forceinline__ void bc_poll(const unsigned long long* rec_all, int e
                 break;
             }
             if (g < G) {
-                wa = __hip_atomic_load(&rec[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                wb = __hip_atomic_load(&rec[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                wa = rec_ld<XM>(&rec[0]);
+                wb = rec_ld<XM>(&rec[1]);
             }
-            // another block that gave up has set err: its record may never come
-            const int failed = (g == 0) ? __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+            // another block that gave up has set err: its record may never
+            // come (checked every 16th read: at agent scope this load takes
+            // longer than a record read through the XCD's L2)
+            const int failed = (g == 0 && (reads & 15) == 15)
+                                   ? __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
             const bool ok = g >= G || ((((wa ^ pb) | (wb ^ pb)) & BC_PAR) == 0);
             if (__shfl(failed, 0) != 0) { fail = 1; break; }
             if (__all(ok)) break;
@@ -495,7 +566,7 @@ __device__ __forceinline__ void bc_poll(const unsigned long long* rec_all, int e
         if (w >= 0 && g == w) { *s_p = idx; *s_sp = slot; }
     }
     __syncthreads();
-    if constexpr (!NF) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if constexpr (XM == XM_FENCE) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 // Entry (r, c) of the stale matrix in slot coordinates (upper triangle stored).
@@ -512,11 +583,10 @@ __device__ __forceinline__ bool bc_better(double b, int i, double bb, int bi) {
 // epoch e: with candidate rows, its entries U_j, V_j (j < cnt) and its value
 // (V_cnt) first; then, once this block's exchange stores have landed, the
 // record.
-template <typename T, int NB>
+template <typename T, int NB, int XM>
 __device__ __forceinline__ void bc_publish(double best, int bi, T val, int lg, int x, bool own, int cnt, int e,
-                                           const BPCoop<T>& co, const T* Ps, double* r_best, int* r_idx,
-                                           int* r_t, int* r_slot) {
-    constexpr bool NF = panel_nf<T>();
+                                           int rank, const BPCoop<T>& co, const T* Ps, double* r_best,
+                                           int* r_idx, int* r_t, int* r_slot) {
     constexpr bool CAND = panel_cand<T>();
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const double b = wave_max(best);                                    // inputs >= -1, no NaN
@@ -528,20 +598,20 @@ __device__ __forceinline__ void bc_publish(double best, int bi, T val, int lg, i
     for (int u = 1; u < BC_WAVES; ++u)
         if (bc_better(r_best[u], r_idx[u], gb, gi)) { gb = r_best[u]; gi = r_idx[u]; }
     const bool found = gb >= 0.0;
-    T* cand = co.cand + ((int64_t)(e % 3) * BC_MAXG + blockIdx.x) * 2 * NB;
+    T* cand = co.cand + ((int64_t)(e % 3) * BC_MAXG + rank) * 2 * NB;
     if (found && own && best >= 0.0 && lg == gi) {
         *r_slot = x;
         if constexpr (CAND) {
             *r_t = t;
-            xb_st<NF>(&cand[2 * cnt + 1], val);
+            xb_st<XM>(&cand[2 * cnt + 1], val);
         }
     }
     if constexpr (CAND) {
         __syncthreads();
-        if (found && t < 2 * cnt) xb_st<NF>(&cand[t], Ps[2 * ((t >> 1) * BC_T + *r_t) + (t & 1)]);
+        if (found && t < 2 * cnt) xb_st<XM>(&cand[t], Ps[2 * ((t >> 1) * BC_T + *r_t) + (t & 1)]);
     }
-    if constexpr (NF) __builtin_amdgcn_s_waitcnt(0);             // this thread's stores have landed
-    else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");      // this thread's stores are visible device-wide
+    if constexpr (XM != XM_FENCE) __builtin_amdgcn_s_waitcnt(0);   // this thread's stores have landed
+    else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");        // this thread's stores are visible device-wide
     __syncthreads();
     if (t == 0) {
         const unsigned long long pb = bc_par(e);
@@ -550,15 +620,23 @@ __device__ __forceinline__ void bc_publish(double best, int bi, T val, int lg, i
             wa = (unsigned long long)__double_as_longlong(gb) | pb;   // gb >= +0: sign bit clear
             wb = pb | ((unsigned long long)(unsigned)gi << 32) | (unsigned long long)(unsigned)*r_slot;
         }
-        unsigned long long* rec = co.rec + ((int64_t)(e % 3) * BC_MAXG + blockIdx.x) * 2;
-        __hip_atomic_store(&rec[0], wa, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&rec[1], wb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned long long* rec = co.rec + ((int64_t)(e % 3) * BC_MAXG + rank) * 2;
+        rec_st<XM>(&rec[0], wa);
+        rec_st<XM>(&rec[1], wb);
     }
 }
 
-template <typename T, int NB>
+// LOCAL: the launch has 8 G blocks; those that find themselves on XCD 0
+// (its id read from the hardware) claim the panel ranks 0 .. G-1 from a
+// counter of this launch, the rest return at once.  Workgroups go to the 8
+// XCDs round-robin (from wherever the dispatch starts), so XCD 0 receives
+// exactly G of them, and the G share its L2, through which they exchange
+// (XM_XCD).  Were fewer to land there, a rank would stay unclaimed and the
+// poll would time out (NaN, reported by pfh_stream_status), never a wrong
+// value.
+template <typename T, int NB, bool LOCAL>
 __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int k0, int np, int epoch0) {
-    constexpr bool NF = panel_nf<T>();
+    constexpr int XM = panel_xm<T, LOCAL>();
     extern __shared__ unsigned char smem_raw[];
     // Panel entries of the own slot, interleaved so that one 16-byte LDS read
     // returns a pair: U_j at Ps[2*(j*BC_T + t)], V_j at Ps[2*(j*BC_T + t) + 1].
@@ -574,18 +652,28 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
     __shared__ int r_idx[BC_WAVES], r_t, r_slot;
     __shared__ int s_w, s_p, s_sp, s_fail;
 
+    __shared__ int s_rank;
+    int rank = (int)blockIdx.x;
+    const int G = LOCAL ? (int)(gridDim.x >> 3) : (int)gridDim.x;
+    if constexpr (LOCAL) {
+        if (__builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)) != 0) return;   // HW_REG_XCC_ID[3:0]
+        if (threadIdx.x == 0) s_rank = atomicAdd(&a.claim[k0 / (2 * NB)], 1);
+        __syncthreads();
+        rank = s_rank;
+        if (rank >= G) return;
+    }
     BPState<T>* st = a.st;
     if (st->dead || st->err) return;            // an earlier panel gave up: the output is NaN already
     const int64_t n = a.n;
     const int t = threadIdx.x;
-    const int x = k0 + blockIdx.x * BC_T + t;   // own slot
+    const int x = k0 + rank * BC_T + t;         // own slot
     const bool own = x < n;
     T* PU = a.P;                                // U_j[x] at PU[j*n + x]
     T* PV = a.P + (int64_t)NB * n;              // V_j[x] at PV[j*n + x]
 
     double lm = 0.0, ph_re = 0.0, ph_im = 0.0;   // pivot log / phase, kept by block 0 thread 0
     int le = 0;
-    const bool keeper = blockIdx.x == 0 && t == 0;
+    const bool keeper = rank == 0 && t == 0;
     if (keeper) { lm = st->lm; le = st->le; ph_re = st->ph_re; ph_im = st->ph_im; }
 
     for (int q = t; q < 2 * NB + 2; q += BC_T) Qpan[q] = k0 + q;
@@ -600,11 +688,11 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         if (own && x > k0) {
             r = a.W[(int64_t)k0 * n + x];
             Ps[2 * t + 1] = r;
-            xb_st<NF>(&PV[x], r);
+            xb_st<XM>(&PV[x], r);
             const double m = (double)mag2(r);
             if (m == m) best = m;
         }
-        bc_publish<T, NB>(best, lg, r, lg, x, own, 0, epoch0 + 1, co, Ps, r_best, r_idx, &r_t, &r_slot);
+        bc_publish<T, NB, XM>(best, lg, r, lg, x, own, 0, epoch0 + 1, rank, co, Ps, r_best, r_idx, &r_t, &r_slot);
     }
 
     for (int i = 0; i < np; ++i) {
@@ -613,7 +701,7 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         const bool has_next = i + 1 < np;       // row k+2 is this panel's next pivot row
         const int sa = Qpan[k + 1 - k0];
         const int sb = has_next ? Qpan[k + 2 - k0] : -1;
-        bc_poll<NF>(co.rec, e, a.spin, &st->err, &s_w, &s_p, &s_sp, &s_fail);
+        bc_poll<XM>(co.rec, e, G, a.spin, &st->err, &s_w, &s_p, &s_sp, &s_fail);
         if (s_fail) return;                                // err is set: the result will be NaN
         const int wb = s_w;
         int p, sp;
@@ -641,7 +729,7 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
             const T* w0 = panel_cand<T>()
                 ? co.cand + ((int64_t)(e % 3) * BC_MAXG + (wb >= 0 ? wb : 0)) * 2 * NB + t
                 : &src[sp];
-            const T v0 = xb_ld<NF>(w0), v1 = xb_ld<NF>(&src[sa]), v2 = xb_ld<NF>(&src[sbs]);
+            const T v0 = xb_ld<XM>(w0), v1 = xb_ld<XM>(&src[sa]), v2 = xb_ld<XM>(&src[sbs]);
             Bx[0][t] = psel(ok, v0, zero<T>());
             Bx[1][t] = psel(ok, v1, zero<T>());
             Bx[2][t] = psel(ok && sb >= 0, v2, zero<T>());
@@ -717,7 +805,7 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         T u = zero<T>();
         if (act1) {
             u = (w1 + r1) * inv;
-            xb_st<NF>(&PU[(int64_t)i * n + x], u);
+            xb_st<XM>(&PU[(int64_t)i * n + x], u);
         }
         Ps[2 * (i * BC_T + t)] = u;
         if (!has_next) break;
@@ -729,12 +817,12 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         T r = zero<T>();
         if (own && lg >= k + 3) {
             r = w2 + rl + u_kl * Ps[2 * (i * BC_T + t) + 1] - v_kl * u;
-            xb_st<NF>(&PV[(int64_t)(i + 1) * n + x], r);
+            xb_st<XM>(&PV[(int64_t)(i + 1) * n + x], r);
             const double m = (double)mag2(r);
             if (m == m) best = m;
         }
         Ps[2 * ((i + 1) * BC_T + t) + 1] = r;
-        bc_publish<T, NB>(best, lg, r, lg, x, own, i + 1, e + 1, co, Ps, r_best, r_idx, &r_t, &r_slot);
+        bc_publish<T, NB, XM>(best, lg, r, lg, x, own, i + 1, e + 1, rank, co, Ps, r_best, r_idx, &r_t, &r_slot);
     }
 
     if (own) a.perm[lg] = x;   // logical -> slot, for the trailing update
@@ -1071,7 +1159,7 @@ static size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 
 template <typename T>
 struct BPLayout {
-    size_t w0, w1, p, r, perm, pv, pi, st, rec, cand, total;
+    size_t w0, w1, p, r, perm, pv, pi, st, claim, rec, cand, total;
 };
 
 template <typename T>
@@ -1087,6 +1175,7 @@ static BPLayout<T> bp_layout(int64_t n, int nb) {
     L.pv = o; o = al(o + sizeof(double) * nblk);
     L.pi = o; o = al(o + sizeof(int) * nblk);
     L.st = o; o = al(o + sizeof(BPState<T>));
+    L.claim = o; o = al(o + sizeof(int) * (size_t)(n / (2 * nb) + 2));
     L.rec = o; o = al(o + sizeof(unsigned long long) * 3 * BC_MAXG * 2);
     L.cand = o; o = al(o + sizeof(T) * 3 * BC_MAXG * 2 * nb);
     L.total = o;
@@ -1115,29 +1204,61 @@ static bool coop_enabled() {
     }) == 1;
 }
 
-template <typename T, int NB>
-static hipError_t launch_panel(const BPArgs<T>& a, const BPCoop<T>& co, int k0, int np, int epoch0, int G,
-                               hipStream_t s) {
+// Largest panel grid that the XCD-local form takes: one XCD's CUs (the
+// blocks hold one CU each for their LDS), 0 when the form is off.  By
+// measurement it is on for complex types and off for real ones (ms, XCD-local
+// / device-wide: c128 N = 4096 13.4 / 15.7; f64 N = 4096 12.1 / 9.9,
+// N = 8192 30.5 / 27.0, gradient N = 2048 7.2 / 5.9).  Complex types gain
+// because the device-wide form exchanges them with L2 write-back and
+// invalidate fences every pair step; real types already exchange through
+// agent-scope atomics without fences, and lose because the panel's strided
+// reads of W's columns (one cache line per entry) then all go through one
+// XCD's L2 instead of eight.  PFH_PANEL_XCD=0/1 or PFH_TUNE_PANEL_XCD set
+// it for every type.  Read per call.
+static int panel_xcd_max_grid(bool cplx) {
+    const int tv = tune_value(PFH_TUNE_PANEL_XCD);
+    bool on = cplx;
+    if (tv >= 0) {
+        on = tv == 1;
+    } else {
+        const char* e = std::getenv("PFH_PANEL_XCD");
+        if (e && (e[0] == '0' || e[0] == '1')) on = e[0] == '1';
+    }
+    if (!on) return 0;
+    const int cus = device_cu_count();
+    return (cus % 8 == 0) ? cus / 8 : 0;
+}
+
+template <typename T, int NB, bool LOCAL>
+static hipError_t launch_panel_form(const BPArgs<T>& a, const BPCoop<T>& co, int k0, int np, int epoch0, int G,
+                                    hipStream_t s) {
     const size_t smem = sizeof(T) * 2 * NB * BC_T;
     static DevCache attr;   // the LDS limit is set on the device
     hipError_t err = hipSuccess;
     if (!attr.get([&] {
-            err = hipFuncSetAttribute((const void*)bp_panel<T, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)smem);
+            err = hipFuncSetAttribute((const void*)bp_panel<T, NB, LOCAL>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
             return err == hipSuccess ? 1 : 0;
         }))
         return err;
-    // A plain launch: G <= BC_MAXG blocks at one per CU (their LDS) on a
-    // device with more CUs than that are resident together once the
-    // stream's earlier work has drained.  A block that cannot become
-    // resident (the CUs held by other work for seconds) ends in the
-    // barrier's bounded spin, which sets the error flag: the result is NaN
-    // and pfh_stream_status reports PFH_ETIMEOUT; nothing hangs.
-    // (A cooperative launch, which asks the runtime for co-residency, was
-    // measured slower, 13.5-20.5 ms against 12.6 ms at N = 4096, and was
-    // removed.)
-    hipLaunchKernelGGL((bp_panel<T, NB>), dim3(G), dim3(BC_T), smem, s, a, co, k0, np, epoch0);
+    // A plain launch: the G working blocks, at one per CU (their LDS), are
+    // resident together once the stream's earlier work has drained.  A
+    // block that cannot become resident (the CUs held by other work for
+    // seconds) ends in the barrier's bounded spin, which sets the error
+    // flag: the result is NaN and pfh_stream_status reports PFH_ETIMEOUT;
+    // nothing hangs.  (A cooperative launch, which asks the runtime for
+    // co-residency, was measured slower, 13.5-20.5 ms against 12.6 ms at
+    // N = 4096, and was removed.)
+    const int grid = LOCAL ? 8 * G : G;
+    hipLaunchKernelGGL((bp_panel<T, NB, LOCAL>), dim3(grid), dim3(BC_T), smem, s, a, co, k0, np, epoch0);
     return hipGetLastError();
+}
+
+template <typename T, int NB>
+static hipError_t launch_panel(const BPArgs<T>& a, const BPCoop<T>& co, int k0, int np, int epoch0, int G,
+                               hipStream_t s) {
+    if (G <= panel_xcd_max_grid(is_cplx<T>::value)) return launch_panel_form<T, NB, true>(a, co, k0, np, epoch0, G, s);
+    return launch_panel_form<T, NB, false>(a, co, k0, np, epoch0, G, s);
 }
 
 int blocked_take_status(hipStream_t s, int* timed_out) {
@@ -1182,6 +1303,7 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
     a.part_v = (double*)(base + L.pv);
     a.part_i = (int*)(base + L.pi);
     a.st = (BPState<T>*)(base + L.st);
+    a.claim = (int*)(base + L.claim);
     a.n = n;
     a.nb = nb;
     a.spin = tune_value(PFH_TUNE_PANEL_SPIN) >= 0 ? tune_value(PFH_TUNE_PANEL_SPIN) : (1 << 22);
@@ -1743,6 +1865,7 @@ int blocked_grad(const T* A, int64_t n, void* ws, double* log_out, T* phase_out,
     a.part_v = (double*)(base + L.bp.pv);
     a.part_i = (int*)(base + L.bp.pi);
     a.st = (BPState<T>*)(base + L.bp.st);
+    a.claim = (int*)(base + L.bp.claim);
     a.n = n;
     a.nb = NB;
     a.spin = tune_value(PFH_TUNE_PANEL_SPIN) >= 0 ? tune_value(PFH_TUNE_PANEL_SPIN) : (1 << 22);
