@@ -109,10 +109,10 @@ def test_set_get_tuning(lib):
         assert lib.pfh_set_tuning(9, (1 << 22) + 1) == -1 and lib.pfh_set_tuning(9, -2) == -1
         assert lib.pfh_set_tuning(10, 1) == 0 and lib.pfh_set_tuning(10, 2) == -1   # panel exchange form
         assert lib.pfh_set_tuning(11, 0) == -1 and lib.pfh_set_tuning(-1, 0) == -1
-        assert lib.pfh_get_tuning(10, ctypes.byref(v)) == -1
+        assert lib.pfh_get_tuning(11, ctypes.byref(v)) == -1
         assert lib.pfh_get_tuning(0, None) == -1
     finally:
-        for k in range(10):
+        for k in range(11):
             lib.pfh_set_tuning(k, -1)
 
 
