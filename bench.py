@@ -5,17 +5,21 @@ skew-symmetric matrices, N drawn uniformly from the even numbers 16..256,
 entries N(0,1) skew-symmetrised A = (B - B^T)/2, seeded, generated on the GPU
 and resident in HBM before timing.  One step = one pfh_slog_pfaffian_ragged
 call over the whole batch (device-side size bucketing + one persistent launch
-per bucket).  Prints one JSON line.
+per bucket), the matrices packed back to back in input order.  Prints one
+JSON line.
 
 Also reported: the small batched product path (1024 x 32x32 f64, eager and
 replayed from a captured graph, and the same kernel over 2^20 matrices); the
 dominant kernel's roofline fraction (HIP events on the launch stream) with
-its HBM bytes per launch from the committed rocprofv3 counter summary;
-single large matrices (N = 4096 f64 and c128, plain vs blocked; the N = 8192
-precision matrix Q diag(s_i J) Q^T against its closed form); the gradient
+its HBM bytes per launch from the committed rocprofv3 counter summary; the
+whole batch with one production-kernel launch per matrix; single large
+matrices (N = 4096 f64 and c128, plain vs blocked, against the threaded
+oracle; the N = 8192 precision matrix Q diag(s_i J) Q^T, FP64 blocked and
+split-precision, against the oracle and its closed form); the gradient
 1/2 A^-T (512 x 128^2 and 2048^2); the accuracy of a sample against the FP64
-CPU oracle and the oracle's own throughput over the whole batch
-(cpu_baseline, about 10 s of host time).  --profile runs only the ragged
+CPU oracle and the oracle's own throughput over the whole batch on every
+host core this process may use (cpu_baseline, about 10 s), with the oracle's
+single-core time for one N = 4096 matrix.  --profile runs only the ragged
 steps plus a 1 GiB device copy, the workload of the committed profiles.
 """
 from __future__ import annotations
@@ -37,20 +41,18 @@ HBM_PEAK = 8.0e12        # MI355X HBM3E bytes/s
 
 
 def gen_ragged(torch, count, lo, hi, seed, dtype, dev):
-    """Sizes uniform over even lo..hi; data grouped by size on the device."""
+    """Sizes uniform over even lo..hi; matrix i at the next offset in input
+    order (sizes mixed, as a caller's uneven batch comes), generated on the
+    device one size at a time and scattered to its place."""
     rng = np.random.default_rng(seed)
     sizes = (2 * rng.integers(lo // 2, hi // 2 + 1, size=count)).astype(np.int32)
     offsets = np.zeros(count, np.int64)
-    order = np.argsort(sizes, kind="stable")
-    pos = 0
-    for i in order:
-        offsets[i] = pos
-        pos += int(sizes[i]) ** 2
-    data = torch.empty(pos, dtype=dtype, device=dev)
+    offsets[1:] = np.cumsum(sizes.astype(np.int64) ** 2)[:-1]
+    total = int(offsets[-1] + int(sizes[-1]) ** 2)
+    data = torch.empty(total, dtype=dtype, device=dev)
     g = torch.Generator(device=dev)
     for n in np.unique(sizes):
-        idx = order[sizes[order] == n]
-        base = int(offsets[idx[0]])
+        idx = np.nonzero(sizes == n)[0]
         k = len(idx)
         g.manual_seed(seed * 1000003 + int(n))
         if dtype.is_complex:
@@ -59,8 +61,29 @@ def gen_ragged(torch, count, lo, hi, seed, dtype, dev):
                               torch.randn(k, n, n, generator=g, device=dev, dtype=rdt)) / math.sqrt(2)
         else:
             B = torch.randn(k, n, n, generator=g, device=dev, dtype=dtype)
-        data[base:base + k * n * n] = ((B - B.transpose(1, 2)) / 2).reshape(-1)
+        pos = (torch.from_numpy(offsets[idx]).to(dev)[:, None]
+               + torch.arange(int(n) * int(n), device=dev)[None, :]).reshape(-1)
+        data[pos] = ((B - B.transpose(1, 2)) / 2).reshape(-1)
+        del B, pos
     return data, offsets, sizes
+
+
+def host_cpu():
+    """(threads this process may use, nproc, CPU model) of the host."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return usable, os.cpu_count() or usable, model
 
 
 class Ragged:
@@ -200,38 +223,35 @@ def main():
     big_flops = float(np.sum(sizes[big].astype(np.float64) ** 3) / 3.0)
     achieved = big_flops / t_big
 
-    # the plain all-FP64 GPU form, one matrix per call (the reference's
-    # per-matrix driver: PFH_METHOD_UNBLOCKED, three launches per pair step),
-    # timed on a sample of the batch and extrapolated to it
+    # per-matrix launches of the production kernels: the whole batch, one
+    # pfh_slog_pfaffian_f64 call (batch 1, the automatic method) per matrix,
+    # as a caller without the ragged entry point would run it
     import ctypes
-    plain_res = None
-    k_pl = min(96, args.count)
-    pl_idx = np.random.default_rng(args.seed + 3).choice(args.count, size=k_pl, replace=False)
     need = ctypes.c_size_t(0)
-    lib.pfh_slog_pfaffian_workspace_size(_lib.PFH_F64, 1, 256, _lib.PFH_METHOD_UNBLOCKED, need)
-    ws_pl = torch.empty(max(int(need.value), 1), dtype=torch.uint8, device=dev)
-    lo_pl = torch.empty(1, dtype=torch.float64, device=dev)
-    ph_pl = torch.empty(1, dtype=torch.float64, device=dev)
+    lib.pfh_slog_pfaffian_workspace_size(_lib.PFH_F64, 1, int(sizes.max()), _lib.PFH_METHOD_AUTO, need)
+    ws_pm = torch.empty(max(int(need.value), 1), dtype=torch.uint8, device=dev)
+    lo_pm = torch.empty(args.count, dtype=torch.float64, device=dev)
+    ph_pm = torch.empty(args.count, dtype=torch.float64, device=dev)
+    base_ptr = data.data_ptr()
+    calls = [(base_ptr + 8 * int(offsets[i]), int(sizes[i]), lo_pm.data_ptr() + 8 * i, ph_pm.data_ptr() + 8 * i)
+             for i in range(args.count)]
+    fn_pm = lib.pfh_slog_pfaffian_f64
 
-    def plain_sample():
-        for i in pl_idx:
-            n_i = int(sizes[i])
-            st = lib.pfh_slog_pfaffian_method(_lib.PFH_F64, data.data_ptr() + 8 * int(offsets[i]), 1, n_i,
-                                              _lib.PFH_METHOD_UNBLOCKED, lo_pl.data_ptr(), ph_pl.data_ptr(),
-                                              ws_pl.data_ptr(), ws_pl.numel(), strm)
+    def per_matrix():
+        for a_ptr, n_i, l_ptr, p_ptr in calls:
+            st = fn_pm(a_ptr, 1, n_i, l_ptr, p_ptr, ws_pm.data_ptr(), ws_pm.numel(), strm)
             if st != 0:
-                raise RuntimeError(f"plain slog status {st}")
-    plain_sample()
+                raise RuntimeError(f"per-matrix slog status {st}")
+    per_matrix()
     torch.cuda.synchronize()
-    t_pl = time_events(torch, plain_sample, 1)
-    # extrapolate by the sample's share of the batch's pair steps
-    steps_all = float(np.sum(sizes // 2))
-    steps_pl = float(np.sum(sizes[pl_idx] // 2))
-    t_plain_batch = t_pl * steps_all / steps_pl
-    plain_res = {"sample_matrices": int(k_pl), "sample_ms": t_pl * 1e3,
-                 "extrapolated_batch_ms": t_plain_batch * 1e3, "pf_per_s": args.count / t_plain_batch,
-                 "speedup_of_batched": t_plain_batch / (ms / 1e3)}
-    print(f"# plain per-matrix: {plain_res}", file=sys.stderr, flush=True)
+    t_pm = time_events(torch, per_matrix, 1)
+    per_matrix_res = {"ms": t_pm * 1e3, "pf_per_s": args.count / t_pm,
+                      "speedup_of_ragged_call": t_pm / (ms / 1e3),
+                      "equal_to_ragged": bool(torch.equal(lo_pm, run.log) and torch.equal(ph_pm, run.ph)),
+                      "note": "one pfh_slog_pfaffian_f64 launch per matrix (batch 1, automatic method), "
+                              "the same kernels as the ragged call"}
+    print(f"# per-matrix launches: {per_matrix_res}", file=sys.stderr, flush=True)
+    del calls, ws_pm, lo_pm, ph_pm
 
     # the same batch shape in complex128
     datac, offc, sizc = gen_ragged(torch, args.count, 16, 256, args.seed, torch.complex128, dev)
@@ -249,7 +269,7 @@ def main():
         parts = [datac[int(offc[i]):int(offc[i]) + int(sizc[i]) ** 2] for i in idc]
         hostc = torch.cat(parts).cpu().numpy()
         offs_c = np.cumsum([0] + [int(sizc[i]) ** 2 for i in idc[:-1]]).astype(np.int64)
-        wl, wp = oracle.slog_ragged(hostc, offs_c, sizc[idc], threads=max(1, min(16, os.cpu_count() or 1)))
+        wl, wp = oracle.slog_ragged(hostc, offs_c, sizc[idc], threads=host_cpu()[0])
         gl = runc.log.cpu().numpy()[idc]
         gp = runc.ph.cpu().numpy()[idc]
         c128_res["max_abs_dlog"] = float(np.max(np.abs(gl - wl)))
@@ -325,6 +345,7 @@ def main():
         return t, float(lo.item()), complex(ph.item())
 
     large_res = {}
+    cpu_single = None
     for n, dtype, tag in ((4096, torch.float64, "c2_f64_n4096"), (4096, torch.complex128, "c2_c128_n4096")):
         A = large(n, dtype, args.seed + 7)
         tb, lb, pb = time_slog(A, _lib.PFH_METHOD_BLOCKED, 3)
@@ -333,6 +354,22 @@ def main():
         large_res[tag] = {"blocked_ms": tb * 1e3, "unblocked_ms": tu * 1e3, "speedup": tu / tb,
                           "blocked_tflops": fl / tb / 1e12, "dlog_blocked_vs_unblocked": abs(lb - lu),
                           "sign_agree": abs(pb - pu) < 1e-9}
+        if not args.no_cpu:
+            import oracle
+            Ah = A.cpu().numpy()
+            l0, p0 = oracle.slog(Ah, threads=host_cpu()[0])
+            large_res[tag]["dlog_blocked_vs_oracle"] = abs(lb - l0)
+            large_res[tag]["dlog_unblocked_vs_oracle"] = abs(lu - l0)
+            large_res[tag]["phase_agree_with_oracle"] = abs(pb - complex(p0)) < 1e-9
+            if not dtype.is_complex:
+                # the reference's CPU algorithm on one core, one N = 4096 matrix
+                t0 = time.time()
+                l1, _ = oracle.slog(Ah, threads=1)
+                t1 = time.time() - t0
+                cpu_single = {"workload": "one N = 4096 f64 slog (the C2 matrix)", "seconds": t1, "threads": 1,
+                              "kind": "port", "gpu_blocked_ms": tb * 1e3, "gpu_speedup": t1 / tb,
+                              "equal_to_threaded_oracle": l1 == l0}
+            del Ah
         print(f"# {tag}: {large_res[tag]}", file=sys.stderr, flush=True)
         del A
     # C3-shaped input: A = Q diag-block([0, s_i; -s_i, 0]) Q^T, s_i log-uniform in [1e-3, 1e3]
@@ -356,20 +393,32 @@ def main():
     closed = float(torch.log(svals).sum().item())
     tb8, lb8, pb8 = time_slog(A8, _lib.PFH_METHOD_BLOCKED, 2)
     tu8, lu8, pu8 = time_slog(A8, _lib.PFH_METHOD_UNBLOCKED, 1)
+    tm8, lm8, pm8 = time_slog(A8, _lib.PFH_METHOD_MIXED, 2)
+    l08 = None
+    if not args.no_cpu:
+        import oracle
+        l08, s08 = oracle.slog(A8.cpu().numpy(), threads=host_cpu()[0])
     large_res["c3_f64_n8192"] = {"blocked_ms": tb8 * 1e3, "unblocked_ms": tu8 * 1e3, "speedup": tu8 / tb8,
                                  "dlog_blocked_vs_unblocked": abs(lb8 - lu8),
+                                 "dlog_blocked_vs_oracle": None if l08 is None else abs(lb8 - l08),
                                  "dlog_blocked_vs_closed_form": abs(lb8 - closed),
                                  "sign_blocked": pb8.real, "sign_closed_form": sign_det_q,
                                  "note": "closed form log|Pf| = sum log s_i is exact only to ~kappa*u of the rounded Q"}
     print(f"# c3: {large_res['c3_f64_n8192']}", file=sys.stderr, flush=True)
-    # mixed vs FP64 on the same matrix: MIXED runs the FP64 blocked path (the
-    # trailing update stores and streams FP64 W at 8 flop/B, under the FP64
-    # ridge of 9.8 flop/B, so a lower-precision product would not shorten it)
-    tm8, lm8, pm8 = time_slog(A8, _lib.PFH_METHOD_MIXED, 2)
-    large_res["mixed_vs_fp64_n8192"] = {"mixed_ms": tm8 * 1e3, "fp64_blocked_ms": tb8 * 1e3,
-                                        "dlog": abs(lm8 - lb8), "sign_agree": abs(pm8 - pb8) < 1e-12,
-                                        "within_1e-9": abs(lm8 - lb8) <= 1e-9,
-                                        "note": "MIXED executes the FP64 blocked path"}
+    # the split-precision method on the same matrix: FP64 panel, trailing
+    # update as bf16 digit products (exact in fp32) recombined in FP64
+    slices = ctypes.c_int(-1)
+    lib.pfh_get_tuning(_lib.TUNE_KEYS["mixed_slices"], ctypes.byref(slices))
+    large_res["mixed_vs_fp64_n8192"] = {
+        "mixed_ms": tm8 * 1e3, "fp64_blocked_ms": tb8 * 1e3, "mixed_over_fp64_time": tm8 / tb8,
+        "slices": slices.value if slices.value > 0 else "built-in (9)",
+        "dlog_mixed_vs_oracle": None if l08 is None else abs(lm8 - l08),
+        "dlog_fp64_vs_oracle": None if l08 is None else abs(lb8 - l08),
+        "dlog_mixed_vs_fp64": abs(lm8 - lb8),
+        "sign_mixed_equals_oracle": None if l08 is None else pm8.real == s08,
+        "within_1e-9": None if l08 is None else abs(lm8 - l08) <= 1e-9,
+        "note": "MIXED: FP64 panel; trailing update on v_mfma_f32_16x16x32_bf16 with 7-bit digit slices, "
+                "exact fp32 sums, FP64 recombination"}
     print(f"# mixed: {large_res['mixed_vs_fp64_n8192']}", file=sys.stderr, flush=True)
     del A8
 
@@ -424,7 +473,7 @@ def main():
         idx = np.arange(k)
         host = data.cpu().numpy()
         sub_sizes = sizes[idx]
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads, nproc, model = host_cpu()
         # correctness sample: the first k matrices against the oracle
         wl, ws = oracle.slog_ragged(host, offsets[idx], sub_sizes, threads=threads)
         gl = run.log.cpu().numpy()[idx]
@@ -440,8 +489,11 @@ def main():
             if tcpu >= 10.0:
                 break
         cpu_baseline = {"value": passes * args.count / tcpu, "unit": "Pf/s", "cores": threads, "kind": "port",
+                        "nproc": nproc, "cpu_model": model,
                         "sample": f"{passes} pass(es) over the full {args.count}-matrix batch (N 16-256 f64), "
-                                  f"{threads} threads, one matrix per task, {tcpu:.1f} s"}
+                                  f"{threads} threads (every core this process may use), one matrix per task, "
+                                  f"{tcpu:.1f} s",
+                        "single_matrix": cpu_single}
 
     line = {
         "metric": "batched Pf/s (N 16-256); slog_pf N=8192 ms; mixed vs FP64, |dlog|<=1e-9",
@@ -449,7 +501,9 @@ def main():
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seeded N(0,1), A=(B-B^T)/2, generated on GPU)",
         "config": {"workload": "ragged slog_pfaffian: 16384 f64 skew matrices, even N uniform in [16,256]",
-                   "count": args.count, "seed": args.seed, "parallelism": "single GPU"},
+                   "count": args.count, "seed": args.seed, "parallelism": "single GPU",
+                   "packing": "one buffer, matrices back to back in input order (sizes mixed); "
+                              "the call buckets them by size on the device"},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK,
                      "traffic": committed_traffic("pf_bat_kernel<256,12>"),
@@ -459,7 +513,7 @@ def main():
                         "frac_fp64_peak": flops / (ms / 1e3) / FP64_PEAK, "triangle_bytes": tri_bytes},
         "graph_replay": {"ms_per_step": ms_graph, "pf_per_s": args.count / (ms_graph / 1e3),
                          "equal_to_eager": graph_ok},
-        "plain_per_matrix_f64": plain_res,
+        "per_matrix_launches_f64": per_matrix_res,
         "ragged_c128": c128_res,
         "small_batched_c1": {"workload": "pfaffian() 1024 x 32x32 f64", "ms": t_c1 * 1e3,
                              "pf_per_s": 1024 / t_c1,

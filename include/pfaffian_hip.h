@@ -37,7 +37,8 @@ extern "C" {
 #define PFH_METHOD_AUTO 0
 #define PFH_METHOD_UNBLOCKED 1   /* plain all-FP64 path, one rank-2 update per pair step */
 #define PFH_METHOD_BLOCKED 2     /* blocked, FP64 trailing update */
-#define PFH_METHOD_MIXED 3       /* reserved for a split-precision trailing update; runs BLOCKED (FP64) today */
+#define PFH_METHOD_MIXED 3       /* blocked, FP64 panel, float64 trailing update on the bf16 matrix cores with
+                                    exact digit products recombined in FP64 (other dtypes: as BLOCKED) */
 
 /* Short text for a status code. */
 const char* pfh_status_string(int status);
@@ -183,7 +184,9 @@ int pfh_pfaffian_cpu_f64(double* A, int32_t n, double* out);
                                        0 sets it at once.  For tests of that path, not a tuning choice */
 #define PFH_TUNE_PANEL_XCD 10      /* blocked panel exchange: 1 through one XCD's L2 when the panel's blocks
                                        fit one XCD, 0 device-wide; built-in: 1 for complex, 0 for real types */
-#define PFH_TUNE_NKEYS 11
+#define PFH_TUNE_MIXED_SLICES 11   /* PFH_METHOD_MIXED: 7-bit digits per value of the split-precision trailing
+                                       update, 5 .. 9 */
+#define PFH_TUNE_NKEYS 12
 
 /* Sets a tuning key (value -1: the built-in choice); PFH_EINVAL for an
  * unknown key or a value outside the key's list.  Takes effect for calls

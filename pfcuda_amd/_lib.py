@@ -22,7 +22,7 @@ PFH_METHOD_AUTO, PFH_METHOD_UNBLOCKED, PFH_METHOD_BLOCKED, PFH_METHOD_MIXED = 0,
 TUNE_KEYS = {
     "f64_bucket_128": 0, "f64_bucket_192": 1, "f64_bucket_256": 2, "c128_bucket_128": 3,
     "trail_f64": 4, "trail_c128": 5, "trail_sweep": 6, "panel_launch": 7, "grad_blocked": 8,
-    "panel_spin": 9, "panel_xcd": 10,
+    "panel_spin": 9, "panel_xcd": 10, "mixed_slices": 11,
 }
 
 _vp = ctypes.c_void_p

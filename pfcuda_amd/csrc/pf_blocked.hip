@@ -38,6 +38,14 @@
 
 namespace pfh {
 
+// Digits per value of the split-precision trailing update, set by the
+// |dlog| <= 1e-9 bound (measured against the FP64 oracle, |dlog| at S = 7 /
+// 8 / 9: N = 8192 precision matrix 1.9e-7 / 3.9e-10 / 1.8e-11, random
+// N = 4096 2.8e-8 / 3.5e-10 / 4.5e-12): S = 8 meets it by a factor 2.5 only,
+// S = 9 by 50.
+#ifndef PFH_MIXED_SLICES_DEFAULT
+#define PFH_MIXED_SLICES_DEFAULT 9
+#endif
 constexpr int BP_THREADS = 128;    // columns per block in the row kernels
 constexpr int BP_TILE = 64;        // trailing-update tile edge
 constexpr int BC_T = 256;          // cooperative panel: threads per block = slots per block
@@ -1123,6 +1131,168 @@ __global__ void __launch_bounds__(256) bp_trailing_staged(const T* __restrict__ 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Split-precision trailing update (PFH_METHOD_MIXED, float64): the product
+// sum_j (U_j V_j' - V_j U_j') on the bf16 matrix cores with exact products,
+// recombined in FP64.  The panel stays FP64.
+//
+// bp_slice writes, for every slot x of the panel vectors and each half h of
+// the K = 64 values (pairs 0-15, 16-31: one MFMA's K range each), a scale
+// E[x][h] (the exponent of the largest of those values, so every one is
+// below 2^E in magnitude) and S signed 7-bit digits of each value:
+//   v = sum_t d_t 2^(E - 7 (t + 1)) + rho,  |d_t| <= 127,  |rho| < 2^(E - 7 S).
+// The digits are small integers, exact in bf16, stored D[t][x][q] (q = 2j:
+// U_j, q = 2j + 1: V_j; 64 per slot).  (One scale per slot over all 64
+// values was measured first: on the N = 8192 precision matrix, whose panel
+// vectors span six decades, its |dlog| was 1.9e-9 at S = 8.)
+//
+// bp_trailing_mixed forms, per 16 x 16 block of a 64 x 64 upper tile, per
+// half h and per digit order d = ta + tb < S,
+//   acc_{h,d} = sum_{ta + tb = d} Dta(A)_h Dtb(B)_h
+// with v_mfma_f32_16x16x32_bf16.  Every product of two digits is exact in
+// fp32 and every partial sum is an integer of magnitude at most
+// S * 32 * 127^2 < 2^24 (S <= 16), so acc_{h,d} is exact whatever the order
+// of the additions.  Then
+//   C = sum_h 2^(E[r][h] + E[c][h]) sum_d 2^(-7 (d + 2)) acc_{h,d}
+// in FP64.  The digit products left out (ta + tb >= S) and the remainders
+// bound the error of an entry by about 32 (S + 1) 2^(-7 S) 2^(E[r][h] +
+// E[c][h]) per half: relative to the largest operand entries of that half,
+// 2^-44 at S = 7, 2^-51 at S = 8, 2^-58 at S = 9 (FP64 products: 2^-53).
+// The B operand is the A-order digits with each pair (U_j, V_j) turned into
+// (V_j, -U_j): a swap of the two bf16 halves of a dword and a sign flip.
+// ---------------------------------------------------------------------------
+typedef __bf16 mx_bf8 __attribute__((ext_vector_type(8)));
+typedef float mx_f4 __attribute__((ext_vector_type(4)));
+constexpr int MX_K = 64;   // 2 nb at nb = 32
+
+// one wave per slot x in [x0, x1): lane q holds value q (A order)
+template <int S>
+__global__ void __launch_bounds__(256) bp_slice(const double* __restrict__ P, int64_t n, int nb, int np, int x0,
+                                                int x1, __bf16* __restrict__ D, int* __restrict__ E) {
+    const int x = x0 + (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (x >= x1) return;
+    const int q = threadIdx.x & 63, j = q >> 1;
+    const double a = (j < np) ? P[(int64_t)((q & 1) ? nb + j : j) * n + x] : 0.0;
+    double m = fabs(a);   // the largest of this lane's half (32 lanes)
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    int e = 0;
+    if (m > 0.0 && m <= 1.7976931348623157e308) (void)frexp(m, &e);   // |a| < 2^e; NaN / inf keep 0 and propagate
+    double v = ldexp(a, -e);                                          // |v| < 1
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        v *= 128.0;
+        const double dgt = trunc(v);                                  // |dgt| <= 127, exact
+        v -= dgt;                                                     // exact
+        D[((int64_t)t * n + x) * MX_K + q] = (__bf16)(float)dgt;
+    }
+    if ((q & 31) == 0) E[2 * x + (q >> 5)] = e;
+}
+
+// (U, V) pairs of a chunk of 8 A-order digits -> (V, -U): B-order digits
+__device__ __forceinline__ mx_bf8 mx_to_b(mx_bf8 a) {
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    u4 w;
+    __builtin_memcpy(&w, &a, 16);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = ((w[i] >> 16) | (w[i] << 16)) ^ 0x80000000u;   // low half V, high half -U
+    mx_bf8 b;
+    __builtin_memcpy(&b, &w, 16);
+    return b;
+}
+
+template <int S>
+__global__ void __launch_bounds__(256) bp_trailing_mixed(const double* __restrict__ Wo, double* __restrict__ Wn,
+                                                         const __bf16* __restrict__ D, const int* __restrict__ E,
+                                                         const int* __restrict__ perm, int64_t n, int lo,
+                                                         int ntile, int pslot) {
+    int ti, tj;
+    if (!xcd_tile(blockIdx.x, ntile, ti, tj)) return;
+    const int r0 = lo + ti * BP_TILE + 16 * (threadIdx.x >> 6), c0 = lo + tj * BP_TILE;
+    const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+    const int n1 = (int)n - 1;
+    // output rows of this lane: r0 + 4 lq + e; column c0 + 16 u + lr
+    int pr[4], pc[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) pr[e] = perm[min(r0 + 4 * lq + e, n1)];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pc[u] = perm[min(c0 + 16 * u + lr, n1)];
+    const int sa = pslot ? perm[min(r0 + lr, n1)] : min(r0 + lr, n1);   // slot of this lane's A row
+    int sb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) sb[u] = pslot ? pc[u] : min(c0 + 16 * u + lr, n1);
+    int er[4][2], ec[4][2];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int x = pslot ? pr[e] : min(r0 + 4 * lq + e, n1);
+        er[e][0] = E[2 * x];
+        er[e][1] = E[2 * x + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { ec[u][0] = E[2 * sb[u]]; ec[u][1] = E[2 * sb[u] + 1]; }
+    mx_bf8 ad[S][2];
+#pragma unroll
+    for (int t = 0; t < S; ++t)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            ad[t][h] = *reinterpret_cast<const mx_bf8*>(D + ((int64_t)t * n + sa) * MX_K + 32 * h + 8 * lq);
+    double wv[4][4];   // stale entries (r0 + 4 lq + e, c0 + 16 u + lr)
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) wv[u][e] = w_at(Wo, n, pr[e], pc[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        mx_bf8 bd[S][2];
+#pragma unroll
+        for (int t = 0; t < S; ++t)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                bd[t][h] = mx_to_b(*reinterpret_cast<const mx_bf8*>(D + ((int64_t)t * n + sb[u]) * MX_K + 32 * h + 8 * lq));
+        double res[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+        for (int d = 0; d < S; ++d) {
+            mx_f4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ta = 0; ta <= d; ++ta) {
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ad[ta][0], bd[d - ta][0], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ad[ta][1], bd[d - ta][1], acc1, 0, 0, 0);
+            }
+            const double sc = ldexp(1.0, -7 * (d + 2));
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                res[0][e] = fma((double)acc0[e], sc, res[0][e]);
+                res[1][e] = fma((double)acc1[e], sc, res[1][e]);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = r0 + 4 * lq + e, c = c0 + 16 * u + lr;
+            const double cv = ldexp(res[0][e], er[e][0] + ec[u][0]) + ldexp(res[1][e], er[e][1] + ec[u][1]);
+            if (c < n && c > r) Wn[(int64_t)r * n + c] = wv[u][e] + cv;
+        }
+    }
+}
+
+// Digit count of the split-precision update: PFH_TUNE_MIXED_SLICES (5 .. 9)
+// or PFH_MIXED_SLICES, else the built-in choice.
+static int mixed_slices() {
+    const int tv = tune_value(PFH_TUNE_MIXED_SLICES);
+    if (tv >= 5 && tv <= 9) return tv;
+    const char* e = std::getenv("PFH_MIXED_SLICES");
+    if (e && e[0] >= '5' && e[0] <= '9' && e[1] == 0) return e[0] - '0';
+    return PFH_MIXED_SLICES_DEFAULT;
+}
+
+template <int S>
+static void launch_mixed_trailing(const double* Wo, double* Wn, const double* P, __bf16* D, int* E,
+                                  const int* perm, int64_t n, int k0, int lo, int nb, int np, int ntile, int tiles,
+                                  int pslot, hipStream_t s) {
+    const int nslot = (int)(n - k0);
+    hipLaunchKernelGGL(bp_slice<S>, dim3((unsigned)((nslot + 3) / 4)), dim3(256), 0, s, P, n, nb, np, k0, (int)n, D, E);
+    hipLaunchKernelGGL(bp_trailing_mixed<S>, dim3(tiles), dim3(256), 0, s, Wo, Wn, D, E, perm, n, lo, ntile, pslot);
+}
+
 // Which form runs, by measurement (ms, staged / register): the f64
 // factorization keeps the register form (N = 4096: 14.2 / 13.9, N = 8192:
 // 35.6 / 34.8); its c128 form (N = 4096: 21.7 / 22.3) and the block sweep
@@ -1159,11 +1329,11 @@ static size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 
 template <typename T>
 struct BPLayout {
-    size_t w0, w1, p, r, perm, pv, pi, st, claim, rec, cand, total;
+    size_t w0, w1, p, r, perm, pv, pi, st, claim, rec, cand, mxd, mxe, total;
 };
 
 template <typename T>
-static BPLayout<T> bp_layout(int64_t n, int nb) {
+static BPLayout<T> bp_layout(int64_t n, int nb, bool mixed = false) {
     BPLayout<T> L{};
     const int64_t nblk = (n + BP_THREADS - 1) / BP_THREADS + 1;
     size_t o = 0;
@@ -1178,6 +1348,11 @@ static BPLayout<T> bp_layout(int64_t n, int nb) {
     L.claim = o; o = al(o + sizeof(int) * (size_t)(n / (2 * nb) + 2));
     L.rec = o; o = al(o + sizeof(unsigned long long) * 3 * BC_MAXG * 2);
     L.cand = o; o = al(o + sizeof(T) * 3 * BC_MAXG * 2 * nb);
+    L.mxd = L.mxe = o;
+    if (mixed) {   // split-precision digits (up to 8 per value) and slot scales
+        L.mxd = o; o = al(o + sizeof(__bf16) * 9 * (size_t)n * MX_K);
+        L.mxe = o; o = al(o + sizeof(int) * 2 * (size_t)n);
+    }
     L.total = o;
     return L;
 }
@@ -1277,11 +1452,10 @@ int blocked_take_status(hipStream_t s, int* timed_out) {
 }
 
 size_t blocked_workspace_bytes(int dtype, int64_t n, bool mixed) {
-    (void)mixed;
     const int nb = blocked_panel_width(dtype);
     switch (dtype) {
         case DT_F32: return bp_layout<float>(n, nb).total;
-        case DT_F64: return bp_layout<double>(n, nb).total;
+        case DT_F64: return bp_layout<double>(n, nb, mixed && nb == 32).total;
         case DT_C64: return bp_layout<c64>(n, nb).total;
         case DT_C128: return bp_layout<c128>(n, nb).total;
     }
@@ -1291,9 +1465,11 @@ size_t blocked_workspace_bytes(int dtype, int64_t n, bool mixed) {
 template <typename T>
 int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* log_out, T* phase_out,
                  hipStream_t s) {
-    (void)mixed;
     if (nb <= 0 || nb % 8 != 0) return (int)hipErrorInvalidValue;   // bp_row reads P in batches of 8 pairs
-    const BPLayout<T> L = bp_layout<T>(n, nb);
+    // the split-precision trailing update serves float64 at nb = 32; other
+    // types run the FP64 (or their own precision's) update under MIXED
+    const bool mx = mixed && std::is_same<T, double>::value && nb == 32;
+    const BPLayout<T> L = bp_layout<T>(n, nb, mx);
     char* base = (char*)ws;
     T* W[2] = {(T*)(base + L.w0), (T*)(base + L.w1)};
     BPArgs<T> a{};
@@ -1355,7 +1531,18 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
                 if (e != hipSuccess) return (int)e;
             }
             if constexpr (std::is_same<T, double>::value) {
-                if (nb == 32 && trail_staged(PFH_TUNE_TRAIL_F64, false)) {
+                if (mx) {
+                    __bf16* D = (__bf16*)(base + L.mxd);
+                    int* E = (int*)(base + L.mxe);
+                    const int pslot = panel_coop ? 1 : 0;
+                    switch (mixed_slices()) {
+                        case 5: launch_mixed_trailing<5>(W[cur], W[cur ^ 1], a.P, D, E, a.perm, n, (int)k0, (int)lo, nb, np, ntile, tiles, pslot, s); break;
+                        case 6: launch_mixed_trailing<6>(W[cur], W[cur ^ 1], a.P, D, E, a.perm, n, (int)k0, (int)lo, nb, np, ntile, tiles, pslot, s); break;
+                        case 7: launch_mixed_trailing<7>(W[cur], W[cur ^ 1], a.P, D, E, a.perm, n, (int)k0, (int)lo, nb, np, ntile, tiles, pslot, s); break;
+                        case 8: launch_mixed_trailing<8>(W[cur], W[cur ^ 1], a.P, D, E, a.perm, n, (int)k0, (int)lo, nb, np, ntile, tiles, pslot, s); break;
+                        default: launch_mixed_trailing<9>(W[cur], W[cur ^ 1], a.P, D, E, a.perm, n, (int)k0, (int)lo, nb, np, ntile, tiles, pslot, s); break;
+                    }
+                } else if (nb == 32 && trail_staged(PFH_TUNE_TRAIL_F64, false)) {
                     hipLaunchKernelGGL((bp_trailing_staged<double, 32>), dim3(tiles), dim3(256), 0, s, W[cur],
                                        W[cur ^ 1], a.P, a.perm, n, (int)lo, np, ntile, panel_coop ? 1 : 0);
                 } else if (nb == 32) {

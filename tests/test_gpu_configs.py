@@ -3,7 +3,8 @@ smaller tests leave open, against the FP64 oracle.
 
 Tolerances (written in each test):
   * slog at N = 4096 / 8192 (f64, c128): |dlog| <= 1e-9, sign equal (f64),
-    |dphase| <= 1e-9 (c128), as the single-matrix configurations state;
+    |dphase| <= 1e-9 (c128), as the single-matrix configurations state; the
+    same for the split-precision method (PFH_METHOD_MIXED);
   * ragged batches: the same bar per matrix;
   * float32 / complex64 slog against the FP64 oracle on the same (rounded)
     input: |dlog| and |dphase| <= 3e-8 n^2 + 1e-5 (single-precision
@@ -95,6 +96,35 @@ def test_c3_n8192_vs_oracle(pf, cuda, method):
     pf.check_status()
     assert abs(float(la) - l0) <= 1e-9, (float(la), l0)
     assert float(s) == s0 == sign_closed
+
+
+def test_c3_n8192_mixed_vs_oracle(pf, cuda):
+    # The split-precision trailing update (bf16 digit products, FP64
+    # recombination) at its default digit count meets the bar, |dlog| <=
+    # 1e-9 with the sign exact; with five digits the same call is off by far
+    # more, so the method really computes in reduced precision.
+    from pfcuda_amd import _lib
+    A, sign_closed, (l0, s0) = _c3_case(cuda)
+    la, s = pf.slog_pfaffian(A, pf.PFH_METHOD_MIXED)
+    assert abs(float(la) - l0) <= 1e-9, (float(la), l0)
+    assert float(s) == s0 == sign_closed
+    key = _lib.TUNE_KEYS["mixed_slices"]
+    try:
+        _lib.check(_lib.lib.pfh_set_tuning(key, 5), "mixed_slices")
+        la5, s5 = pf.slog_pfaffian(A, pf.PFH_METHOD_MIXED)
+    finally:
+        _lib.lib.pfh_set_tuning(key, -1)
+    assert abs(float(la5) - l0) > 1e-7, (float(la5), l0)
+
+
+@pytest.mark.parametrize("n", [300, 640, 1000, 2048])
+def test_mixed_vs_oracle(pf, cuda, n):
+    rng = np.random.default_rng(n + 5)
+    A = oracle.skew_gaussian(rng, n)
+    l0, s0 = oracle.slog(A)
+    la, s = pf.slog_pfaffian(torch.from_numpy(A).to(cuda), pf.PFH_METHOD_MIXED)
+    assert abs(float(la) - l0) <= 1e-9, (n, float(la), l0)
+    assert float(s) == s0
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.complex128])
