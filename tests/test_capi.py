@@ -108,11 +108,13 @@ def test_set_get_tuning(lib):
         assert lib.pfh_set_tuning(9, 0) == 0 and lib.pfh_set_tuning(9, 1 << 22) == 0   # panel spin limit
         assert lib.pfh_set_tuning(9, (1 << 22) + 1) == -1 and lib.pfh_set_tuning(9, -2) == -1
         assert lib.pfh_set_tuning(10, 1) == 0 and lib.pfh_set_tuning(10, 2) == -1   # panel exchange form
-        assert lib.pfh_set_tuning(11, 0) == -1 and lib.pfh_set_tuning(-1, 0) == -1
-        assert lib.pfh_get_tuning(11, ctypes.byref(v)) == -1
+        assert lib.pfh_set_tuning(11, 9) == 0 and lib.pfh_set_tuning(11, 4) == -1   # mixed digit count
+        nkeys = int(re.search(r"#define PFH_TUNE_NKEYS (\d+)", open(HEADER).read()).group(1))
+        assert lib.pfh_set_tuning(nkeys, 0) == -1 and lib.pfh_set_tuning(-1, 0) == -1
+        assert lib.pfh_get_tuning(nkeys, ctypes.byref(v)) == -1
         assert lib.pfh_get_tuning(0, None) == -1
     finally:
-        for k in range(11):
+        for k in range(32):
             lib.pfh_set_tuning(k, -1)
 
 
