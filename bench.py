@@ -120,7 +120,7 @@ def committed_traffic(kernel_key):
     """HBM bytes per launch of the dominant kernel, from the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated against a copy of
     known size); None when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_ragged16384_f64.md")
+    path = os.path.join(ROOT, "profiles", "ragged16384_f64.md")
     try:
         with open(path) as f:
             for line in f:
@@ -142,6 +142,67 @@ def time_events(torch, fn, reps):
     return s.elapsed_time(e) / 1e3 / reps
 
 
+def calibration_copy(torch, dev):
+    """A plain 1 GiB device copy: the byte counters' calibration dispatch."""
+    src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def profile_other(args, torch, pf, _lib, dev):
+    """The --profile large / c1 workloads (no timing, no checks)."""
+    g = torch.Generator(device=dev)
+    if args.profile == "large":
+        for n, dt in ((4096, torch.float64), (4096, torch.complex128)):
+            g.manual_seed(args.seed + 7)
+            if dt.is_complex:
+                X = torch.complex(torch.randn(n, n, generator=g, device=dev, dtype=torch.float64),
+                                  torch.randn(n, n, generator=g, device=dev, dtype=torch.float64)) / math.sqrt(2)
+            else:
+                X = torch.randn(n, n, generator=g, device=dev, dtype=torch.float64)
+            A = ((X - X.T) / 2).contiguous()
+            del X
+            pf.slog_pfaffian(A, _lib.PFH_METHOD_BLOCKED)
+            torch.cuda.synchronize()
+            del A
+        # the same construction built on the host (NumPy): rocBLAS, behind
+        # torch's QR and products, does not initialize under counter collection
+        rng = np.random.default_rng(args.seed + 8)
+        Q, R = np.linalg.qr(rng.standard_normal((8192, 8192)))
+        Q *= np.sign(np.diag(R))
+        del R
+        sv = 10.0 ** (rng.random(4096) * 6 - 3)
+        QD = np.empty_like(Q)
+        QD[:, 0::2] = -Q[:, 1::2] * sv
+        QD[:, 1::2] = Q[:, 0::2] * sv
+        M = QD @ Q.T
+        del Q, QD
+        A8 = torch.from_numpy((M - M.T) / 2).to(dev)
+        del M
+        pf.slog_pfaffian(A8, _lib.PFH_METHOD_BLOCKED)
+        pf.slog_pfaffian(A8, _lib.PFH_METHOD_MIXED)
+        torch.cuda.synchronize()
+        del A8
+        from pfcuda_amd.adjoint import slog_grad
+        g.manual_seed(args.seed + 5)
+        X = torch.randn(2048, 2048, generator=g, device=dev, dtype=torch.float64)
+        slog_grad(((X - X.T) / 2).contiguous())
+        torch.cuda.synchronize()
+    else:
+        for count in (1024, 1 << 20):
+            g.manual_seed(args.seed + 1)
+            B = torch.randn(count, 32, 32, generator=g, device=dev, dtype=torch.float64)
+            A = ((B - B.transpose(1, 2)) / 2).contiguous()
+            del B
+            pf.pfaffian(A)
+            torch.cuda.synchronize()
+            del A
+    calibration_copy(torch, dev)
+    print(f"# profile mode {args.profile}: done, then one 1073741824-byte copy", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,8 +212,12 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile", action="store_true",
-                    help="only the ragged steps plus one 1 GiB device copy (for counter runs)")
+    ap.add_argument("--profile", nargs="?", const="ragged", choices=["ragged", "large", "c1"],
+                    help="a short workload for profiler runs, then one 1 GiB device copy (the byte "
+                         "counters' calibration): ragged = the timed ragged steps; large = one blocked "
+                         "solve each of N = 4096 f64 and c128 and of the N = 8192 precision matrix (FP64 "
+                         "and split precision), and one N = 2048 gradient; c1 = pfaffian() of 1024 and "
+                         "2^20 32x32 f64 matrices")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -168,6 +233,9 @@ def main():
     torch.cuda.set_device(dev)
     lib = _lib.lib
     strm = torch.cuda.current_stream(dev).cuda_stream
+    if args.profile in ("large", "c1"):
+        profile_other(args, torch, pfcuda_amd, _lib, dev)
+        return
     t0 = time.time()
     data, offsets, sizes = gen_ragged(torch, args.count, 16, 256, args.seed, torch.float64, dev)
     torch.cuda.synchronize()
@@ -183,13 +251,9 @@ def main():
     print(f"# ragged: {ms:.3f} ms/step", file=sys.stderr, flush=True)
 
     if args.profile:
-        # calibration kernel for the HBM byte counters: a plain 1 GiB copy
-        src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        print(f"# profile mode: {args.steps} ragged steps and one 1073741824-byte copy done", file=sys.stderr,
-              flush=True)
+        calibration_copy(torch, dev)
+        print(f"# profile mode ragged: {args.steps} ragged steps and one 1073741824-byte copy done",
+              file=sys.stderr, flush=True)
         return
 
     # the same call captured in a graph and replayed (checked against eager)
