@@ -186,7 +186,17 @@ int pfh_pfaffian_cpu_f64(double* A, int32_t n, double* out);
                                        fit one XCD, 0 device-wide; built-in: 1 for complex, 0 for real types */
 #define PFH_TUNE_MIXED_SLICES 11   /* PFH_METHOD_MIXED: 7-bit digits per value of the split-precision trailing
                                        update, 5 .. 9 */
-#define PFH_TUNE_NKEYS 12
+#define PFH_TUNE_BAT_NB_128 12     /* f64 batched blocked kernel, 64 < n <= 128: pairs per panel, 6, 8 or 12 */
+#define PFH_TUNE_BAT_NB_192 13     /* f64, 128 < n <= 192: 6, 8 or 12 */
+#define PFH_TUNE_BAT_NB_256 14     /* f64, 192 < n <= 256: 8, 10 or 12 */
+#define PFH_TUNE_BAT_NB_C128_128 15  /* c128 batched blocked kernel, 64 < n <= 128: 4, 6 or 8 */
+#define PFH_TUNE_BAT_NB_C128_192 16  /* c128, 128 < n <= 192: 4, 6 or 8 */
+#define PFH_TUNE_BAT_NB_C128_256 17  /* c128, 192 < n <= 256: 6 or 8 */
+#define PFH_TUNE_BLOCKED_NB 18     /* f64 blocked factorization: pairs per panel, 16, 24 or 32 */
+#define PFH_TUNE_BLOCKED_NB_C128 19  /* c128 blocked factorization: 8 or 16 */
+#define PFH_TUNE_SWITCH_N 20       /* dense slog of fewer matrices than the device has CUs: the largest n the
+                                      batched kernels take (64, 96, ..., 256); larger ones run blocked */
+#define PFH_TUNE_NKEYS 21
 
 /* Sets a tuning key (value -1: the built-in choice); PFH_EINVAL for an
  * unknown key or a value outside the key's list.  Takes effect for calls

@@ -23,6 +23,9 @@ TUNE_KEYS = {
     "f64_bucket_128": 0, "f64_bucket_192": 1, "f64_bucket_256": 2, "c128_bucket_128": 3,
     "trail_f64": 4, "trail_c128": 5, "trail_sweep": 6, "panel_launch": 7, "grad_blocked": 8,
     "panel_spin": 9, "panel_xcd": 10, "mixed_slices": 11,
+    "bat_nb_128": 12, "bat_nb_192": 13, "bat_nb_256": 14,
+    "bat_nb_c128_128": 15, "bat_nb_c128_192": 16, "bat_nb_c128_256": 17,
+    "blocked_nb": 18, "blocked_nb_c128": 19, "switch_n": 20,
 }
 
 _vp = ctypes.c_void_p

@@ -1,7 +1,8 @@
 """Autotuner for the library's run-time choices (include/pfaffian_hip.h,
 PFH_TUNE_*): which batched kernel serves each slog size bucket, which form of
-the trailing update the blocked paths use, how the blocked panel is launched
-and which sweep computes large gradients.
+the trailing update the blocked paths use, which sweep computes large
+gradients, the panel widths, and the size at which single matrices leave the
+batched kernels for the blocked path.
 
 ``tune()`` times every candidate of every key on the current device, on
 seeded synthetic workloads of the shape the key governs, checks that each
@@ -9,8 +10,10 @@ candidate's results agree with the built-in choice's (|dlog| <= 1e-9, same
 sign or phase within 1e-9), keeps the fastest, and writes a JSON cache keyed
 by the device architecture.  ``load()`` applies a cache written for the same
 architecture; importing ``pfcuda_amd`` with PFH_TUNE_FILE set does the same.
-The panel widths of the batched blocked kernel are compile-time constants
-(-DPFH_BAT_NB_*) and are not tuned here.
+The panel widths (pairs per panel) of the batched blocked kernel's buckets
+and of the blocked factorization choose among the widths the library
+instantiates, and the switch point picks the largest n that a dense call of
+few matrices keeps on the batched kernels (larger ones run blocked).
 
     python -m pfcuda_amd.autotune [--out PATH] [--quick]
 """
@@ -33,9 +36,17 @@ CANDIDATES = {
     "c128_bucket_128": [0, 3],
     "trail_f64": [0, 1], "trail_c128": [0, 1], "trail_sweep": [0, 1],
     "panel_launch": [0], "grad_blocked": [0, 1],
+    # panel widths (pairs per panel) and the single-matrix switch point
+    "bat_nb_128": [6, 8, 12], "bat_nb_192": [6, 8, 12], "bat_nb_256": [8, 10, 12],
+    "bat_nb_c128_128": [4, 6, 8], "bat_nb_c128_192": [4, 6, 8], "bat_nb_c128_256": [6, 8],
+    "blocked_nb": [16, 24, 32], "blocked_nb_c128": [8, 16],
+    "switch_n": [96, 128, 160, 192, 224, 256],
 }
 BUCKETS = {"f64_bucket_128": (64, 128), "f64_bucket_192": (128, 192), "f64_bucket_256": (192, 256),
-           "c128_bucket_128": (64, 128)}
+           "c128_bucket_128": (64, 128),
+           "bat_nb_128": (64, 128), "bat_nb_192": (128, 192), "bat_nb_256": (192, 256),
+           "bat_nb_c128_128": (64, 128), "bat_nb_c128_192": (128, 192), "bat_nb_c128_256": (192, 256)}
+SWITCH_SIZES = (96, 128, 160, 192, 224, 256)
 
 
 def set_tuning(name: str, value: int) -> None:
@@ -87,7 +98,7 @@ def _bucket_workload(torch, name, count, dev):
     gen = torch.Generator(device=dev)
     gen.manual_seed(1234 + hi)
     sizes = [lo + 2 + 2 * int(i) for i in torch.randint(0, (hi - lo) // 2, (count,), generator=torch.Generator().manual_seed(hi))]
-    cplx = name.startswith("c128")
+    cplx = "c128" in name
     mats = [_skew(torch, (n, n), cplx, gen, dev) for n in sizes]
     data, offsets, sz = pack_ragged(mats)
     offsets, szd, mx = offsets.to(dev), sz.to(dev), int(sz.max())   # no host work per call
@@ -97,13 +108,27 @@ def _bucket_workload(torch, name, count, dev):
     return run
 
 
+def _switch_workload(torch, dev):
+    """Single-matrix solves (the automatic method) at n = 96 .. 256, one each:
+    the switch point's candidates differ in which of these run blocked."""
+    from .api import slog_pfaffian
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(5678)
+    mats = [_skew(torch, (n, n), False, gen, dev) for n in SWITCH_SIZES]
+
+    def run():
+        outs = [slog_pfaffian(A) for A in mats]
+        return torch.stack([o[0] for o in outs]), torch.stack([o[1] for o in outs])
+    return run
+
+
 def _single_workload(torch, name, n, dev):
     """One n x n solve (blocked slog) or gradient, through the host API."""
     from .api import slog_pfaffian
     from .adjoint import slog_grad
     gen = torch.Generator(device=dev)
     gen.manual_seed(4321 + n)
-    cplx = name == "trail_c128"
+    cplx = name in ("trail_c128", "blocked_nb_c128")
     A = _skew(torch, (n, n), cplx, gen, dev)
     if name in ("trail_sweep", "grad_blocked"):
         def run():
@@ -140,7 +165,12 @@ def tune(out: str | None = None, quick: bool = False, verbose: bool = True, keys
     choices, timings = {}, {}
     for name in (keys or list(CANDIDATES)):   # the run-time choices (not the panel's spin limit)
         set_tuning(name, -1)
-        run = _bucket_workload(torch, name, count, dev) if name in BUCKETS else _single_workload(torch, name, n_big, dev)
+        if name in BUCKETS:
+            run = _bucket_workload(torch, name, count, dev)
+        elif name == "switch_n":
+            run = _switch_workload(torch, dev)
+        else:
+            run = _single_workload(torch, name, n_big, dev)
         ref = run()
         torch.cuda.synchronize()
         best, best_ms, ms_of = -1, math.inf, {}

@@ -30,7 +30,9 @@
 // at a time with the pivot search of src/pfaffian_cpu.cpp:47-52 (largest
 // magnitude, lowest index on ties, NaN never wins).
 #include <cstdlib>
+#include <type_traits>
 
+#include "../../include/pfaffian_hip.h"
 #include "pf_batch.h"
 #include "pf_types.h"
 #include "pf_wave.h"
@@ -46,7 +48,7 @@
 #define PFH_BAT_NB_128 8
 #endif
 #ifndef PFH_BAT_NB_192
-#define PFH_BAT_NB_192 8
+#define PFH_BAT_NB_192 12   // the autotuner's choice (2048 matrices: 0.57 ms at 12, 0.66 at 8; 16384-batch 3.96 / 4.02 ms)
 #endif
 #ifndef PFH_BAT_NB_256
 #define PFH_BAT_NB_256 12
@@ -398,35 +400,103 @@ int64_t bat_need(int64_t count) {
 
 }  // namespace
 
+// Pairs per panel of each bucket's instantiation: the built-in width
+// (PFH_BAT_NB_*) unless its tuning key names another instantiated one
+// (float64: 6, 8 or 12 up to 192 and 8, 10 or 12 above; complex128: 4, 6
+// or 8, and 6 or 8 above 192).
+static int bat_nb(int dtype, int np) {
+    if (dtype == DT_C128) {
+        const int key = np == 128 ? PFH_TUNE_BAT_NB_C128_128 : np == 192 ? PFH_TUNE_BAT_NB_C128_192
+                                                                          : PFH_TUNE_BAT_NB_C128_256;
+        const int dflt = np == 128 ? PFH_BAT_NB_C128_128 : np == 192 ? PFH_BAT_NB_C128_192 : PFH_BAT_NB_C128_256;
+        const int tv = tune_value(key);
+        if (np < 256 && (tv == 4 || tv == 6 || tv == 8)) return tv;
+        if (np == 256 && (tv == 6 || tv == 8)) return tv;
+        return dflt;
+    }
+    const int key = np == 128 ? PFH_TUNE_BAT_NB_128 : np == 192 ? PFH_TUNE_BAT_NB_192 : PFH_TUNE_BAT_NB_256;
+    const int dflt = np == 128 ? PFH_BAT_NB_128 : np == 192 ? PFH_BAT_NB_192 : PFH_BAT_NB_256;
+    const int tv = tune_value(key);
+    if (np < 256 && (tv == 6 || tv == 8 || tv == 12)) return tv;
+    if (np == 256 && (tv == 8 || tv == 10 || tv == 12)) return tv;
+    return dflt;
+}
+
+// f(std::integral_constant<int, NB>) for the bucket's width nb, over the
+// widths instantiated for (T, NP) (those bat_nb can return)
+template <typename T, int NP, typename F>
+static auto bat_nb_dispatch(int nb, F f) {
+    using I4 = std::integral_constant<int, 4>;
+    using I6 = std::integral_constant<int, 6>;
+    using I8 = std::integral_constant<int, 8>;
+    using I10 = std::integral_constant<int, 10>;
+    using I12 = std::integral_constant<int, 12>;
+    if constexpr (std::is_same<T, double>::value) {
+        if constexpr (NP < 256) {
+            if (nb == 6) return f(I6{});
+            if (nb == 12) return f(I12{});
+            return f(I8{});
+        } else {
+            if (nb == 8) return f(I8{});
+            if (nb == 10) return f(I10{});
+            return f(I12{});
+        }
+    } else {
+        if constexpr (NP < 256) {
+            if (nb == 4) return f(I4{});
+            if (nb == 8) return f(I8{});
+            return f(I6{});
+        } else {
+            if (nb == 6) return f(I6{});
+            return f(I8{});
+        }
+    }
+}
+
+template <typename T, int NP>
+static int64_t bat_need_rt(int dtype, int64_t count) {
+    return bat_nb_dispatch<T, NP>(bat_nb(dtype, NP), [&](auto nbc) -> int64_t {
+        return bat_need<T, NP, decltype(nbc)::value>(count);
+    });
+}
+
+template <typename T, int NP>
+static int bat_launch_rt(int dtype, const BatchDesc& d, hipStream_t s) {
+    return bat_nb_dispatch<T, NP>(bat_nb(dtype, NP), [&](auto nbc) -> int {
+        return bat_launch<T, NP, decltype(nbc)::value>(d, s);
+    });
+}
+
 // doubles of workspace for batches of `count` matrices with n <= maxn: one
 // slab pair (2 NP^2 elements) per resident workgroup of the widest
-// instantiation a bucket can need (the buckets run one after another)
+// instantiation a bucket can need (the buckets run one after another), at
+// the panel widths in force when the size is asked for
 int64_t bat_ws_elems(int dtype, int maxn, int64_t count) {
     if (maxn <= 64 || count <= 0) return 0;
     if (dtype == DT_C128) {
-        int64_t v = bat_need<c128, 128, PFH_BAT_NB_C128_128>(count);
-        if (maxn > 128) { const int64_t b = bat_need<c128, 192, PFH_BAT_NB_C128_192>(count); v = b > v ? b : v; }
-        if (maxn > 192) { const int64_t c = bat_need<c128, 256, PFH_BAT_NB_C128_256>(count); v = c > v ? c : v; }
+        int64_t v = bat_need_rt<c128, 128>(dtype, count);
+        if (maxn > 128) { const int64_t b = bat_need_rt<c128, 192>(dtype, count); v = b > v ? b : v; }
+        if (maxn > 192) { const int64_t c = bat_need_rt<c128, 256>(dtype, count); v = c > v ? c : v; }
         return v;
     }
     if (dtype != DT_F64) return 0;
-    int64_t v = bat_need<double, 128, PFH_BAT_NB_128>(count);
-    if (maxn > 128) { const int64_t b = bat_need<double, 192, PFH_BAT_NB_192>(count); v = b > v ? b : v; }
-    if (maxn > 192) { const int64_t c = bat_need<double, 256, PFH_BAT_NB_256>(count); v = c > v ? c : v; }
+    int64_t v = bat_need_rt<double, 128>(dtype, count);
+    if (maxn > 128) { const int64_t b = bat_need_rt<double, 192>(dtype, count); v = b > v ? b : v; }
+    if (maxn > 192) { const int64_t c = bat_need_rt<double, 256>(dtype, count); v = c > v ? c : v; }
     return v;
 }
 
 int launch_bat(int dtype, const BatchDesc& d, int maxn, hipStream_t s) {
     if (dtype == DT_C128) {
-        if (maxn <= 128) return bat_launch<c128, 128, PFH_BAT_NB_C128_128>(d, s);
-        if (maxn <= 192) return bat_launch<c128, 192, PFH_BAT_NB_C128_192>(d, s);
-        if (maxn <= 256) return bat_launch<c128, 256, PFH_BAT_NB_C128_256>(d, s);
+        if (maxn <= 128) return bat_launch_rt<c128, 128>(dtype, d, s);
+        if (maxn <= 192) return bat_launch_rt<c128, 192>(dtype, d, s);
+        if (maxn <= 256) return bat_launch_rt<c128, 256>(dtype, d, s);
         return (int)hipErrorInvalidValue;
     }
     if (dtype != DT_F64) return (int)hipErrorInvalidValue;
-    if (maxn <= 128) return bat_launch<double, 128, PFH_BAT_NB_128>(d, s);
-    if (maxn <= 192) return bat_launch<double, 192, PFH_BAT_NB_192>(d, s);
-    if (maxn <= 256) return bat_launch<double, 256, PFH_BAT_NB_256>(d, s);
+    if (maxn <= 128) return bat_launch_rt<double, 128>(dtype, d, s);
+    if (maxn <= 192) return bat_launch_rt<double, 192>(dtype, d, s);
+    if (maxn <= 256) return bat_launch_rt<double, 256>(dtype, d, s);
     return (int)hipErrorInvalidValue;
 }
 

@@ -1357,11 +1357,24 @@ static BPLayout<T> bp_layout(int64_t n, int nb, bool mixed = false) {
     return L;
 }
 
-// 32 pair steps per panel; 16 for complex128, whose panel vectors would
-// otherwise not fit the cooperative panel kernel's LDS.
+// Pair steps per panel of the blocked factorization: 32 (float64, and the
+// real / complex64 types), 16 for complex128, whose panel vectors would not
+// fit the panel kernel's LDS at 32; PFH_TUNE_BLOCKED_NB (float64: 16, 24 or
+// 32) and PFH_TUNE_BLOCKED_NB_C128 (8 or 16) choose another instantiated
+// width.  The split-precision method keeps 32 (its digit products take
+// K = 64), and the gradient's block sweep keeps 32 / 16 (grad_panel_width).
 int blocked_panel_width(int dtype) {
-    return dtype == DT_C128 ? 16 : 32;
+    if (dtype == DT_C128) {
+        const int tv = tune_value(PFH_TUNE_BLOCKED_NB_C128);
+        return (tv == 8 || tv == 16) ? tv : 16;
+    }
+    if (dtype == DT_F64) {
+        const int tv = tune_value(PFH_TUNE_BLOCKED_NB);
+        return (tv == 16 || tv == 24 || tv == 32) ? tv : 32;
+    }
+    return 32;
 }
+static int grad_panel_width(int dtype) { return dtype == DT_C128 ? 16 : 32; }
 
 // The panel kernel runs on a device with at least BC_MAXG CUs (so that a
 // panel's blocks, one per CU, are resident together), unless
@@ -1452,7 +1465,7 @@ int blocked_take_status(hipStream_t s, int* timed_out) {
 }
 
 size_t blocked_workspace_bytes(int dtype, int64_t n, bool mixed) {
-    const int nb = blocked_panel_width(dtype);
+    const int nb = (mixed && dtype == DT_F64) ? 32 : blocked_panel_width(dtype);
     switch (dtype) {
         case DT_F32: return bp_layout<float>(n, nb).total;
         case DT_F64: return bp_layout<double>(n, nb, mixed && nb == 32).total;
@@ -1486,7 +1499,9 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
     BPCoop<T> co{};
     co.rec = (unsigned long long*)(base + L.rec);
     co.cand = (T*)(base + L.cand);
-    const bool coop = coop_enabled() && nb == (std::is_same<T, c128>::value ? 16 : 32);
+    const bool coop = coop_enabled() &&
+                      (std::is_same<T, c128>::value ? (nb == 8 || nb == 16)
+                       : std::is_same<T, double>::value ? (nb == 16 || nb == 24 || nb == 32) : nb == 32);
     hipError_t e = hipMemcpyAsync(W[0], A, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return (int)e;
     a.W = W[0];
@@ -1500,8 +1515,16 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
         const int G = (int)((n - k0 + BC_T - 1) / BC_T);
         const bool panel_coop = coop && G <= BC_MAXG;
         if (panel_coop) {
-            if constexpr (std::is_same<T, c128>::value) e = launch_panel<T, 16>(a, co, (int)k0, np, epoch, G, s);
-            else e = launch_panel<T, 32>(a, co, (int)k0, np, epoch, G, s);
+            if constexpr (std::is_same<T, c128>::value) {
+                e = (nb == 8) ? launch_panel<T, 8>(a, co, (int)k0, np, epoch, G, s)
+                              : launch_panel<T, 16>(a, co, (int)k0, np, epoch, G, s);
+            } else if constexpr (std::is_same<T, double>::value) {
+                e = (nb == 16) ? launch_panel<T, 16>(a, co, (int)k0, np, epoch, G, s)
+                  : (nb == 24) ? launch_panel<T, 24>(a, co, (int)k0, np, epoch, G, s)
+                               : launch_panel<T, 32>(a, co, (int)k0, np, epoch, G, s);
+            } else {
+                e = launch_panel<T, 32>(a, co, (int)k0, np, epoch, G, s);
+            }
             if (e != hipSuccess) return (int)e;
             epoch += np;   // one barrier for the first row, one per pair step but the last
         } else {
@@ -1548,6 +1571,12 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
                 } else if (nb == 32) {
                     hipLaunchKernelGGL(bp_trailing_mfma<32>, dim3(tiles), dim3(256), 0, s, W[cur], W[cur ^ 1], a.P,
                                        a.perm, n, (int)lo, np, ntile, panel_coop ? 1 : 0);
+                } else if (nb == 24) {
+                    hipLaunchKernelGGL(bp_trailing_mfma<24>, dim3(tiles), dim3(256), 0, s, W[cur], W[cur ^ 1], a.P,
+                                       a.perm, n, (int)lo, np, ntile, panel_coop ? 1 : 0);
+                } else if (nb == 16) {
+                    hipLaunchKernelGGL(bp_trailing_mfma<16>, dim3(tiles), dim3(256), 0, s, W[cur], W[cur ^ 1], a.P,
+                                       a.perm, n, (int)lo, np, ntile, panel_coop ? 1 : 0);
                 } else {
                     hipLaunchKernelGGL(bp_trailing<T>, dim3(tiles), dim3(256), smem, s, W[cur], W[cur ^ 1], a.P,
                                        a.perm, n, (int)lo, nb, np, ntile, panel_coop ? 1 : 0);
@@ -1555,6 +1584,9 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
             } else if constexpr (std::is_same<T, c128>::value) {
                 if (nb == 16 && trail_staged(PFH_TUNE_TRAIL_C128, true)) {
                     hipLaunchKernelGGL((bp_trailing_staged<c128, 16>), dim3(tiles), dim3(256), 0, s, W[cur],
+                                       W[cur ^ 1], a.P, a.perm, n, (int)lo, np, ntile, panel_coop ? 1 : 0);
+                } else if (nb == 8) {
+                    hipLaunchKernelGGL((bp_trailing_staged<c128, 8>), dim3(tiles), dim3(256), 0, s, W[cur],
                                        W[cur ^ 1], a.P, a.perm, n, (int)lo, np, ntile, panel_coop ? 1 : 0);
                 } else if (nb == 16) {
                     hipLaunchKernelGGL(bp_trailing_mfma_c<16>, dim3(tiles), dim3(256), 0, s, W[cur], W[cur ^ 1],
@@ -2028,7 +2060,7 @@ bool blocked_grad_enabled(int64_t n) {
 }
 
 size_t blocked_grad_workspace_bytes(int dtype, int64_t n) {
-    const int nb = blocked_panel_width(dtype);
+    const int nb = grad_panel_width(dtype);
     switch (dtype) {
         case DT_F32: return gj_layout<float>(n, nb).total;
         case DT_F64: return gj_layout<double>(n, nb).total;

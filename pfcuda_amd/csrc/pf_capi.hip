@@ -43,6 +43,15 @@ bool tune_value_ok(int key, int v) {
         case PFH_TUNE_PANEL_LAUNCH: return v == 0;   // the plain launch is the only form
         case PFH_TUNE_PANEL_SPIN: return v >= 0 && v <= (1 << 22);
         case PFH_TUNE_MIXED_SLICES: return v >= 5 && v <= 9;
+        case PFH_TUNE_BAT_NB_128:
+        case PFH_TUNE_BAT_NB_192: return v == 6 || v == 8 || v == 12;
+        case PFH_TUNE_BAT_NB_256: return v == 8 || v == 10 || v == 12;
+        case PFH_TUNE_BAT_NB_C128_128:
+        case PFH_TUNE_BAT_NB_C128_192: return v == 4 || v == 6 || v == 8;
+        case PFH_TUNE_BAT_NB_C128_256: return v == 6 || v == 8;
+        case PFH_TUNE_BLOCKED_NB: return v == 16 || v == 24 || v == 32;
+        case PFH_TUNE_BLOCKED_NB_C128: return v == 8 || v == 16;
+        case PFH_TUNE_SWITCH_N: return v >= 64 && v <= 256 && v % 32 == 0;
     }
     return false;
 }
@@ -307,11 +316,24 @@ size_t dense_spill_bytes(int dt, int32_t n) {
 }
 size_t dense_ws_bytes(int dt, int64_t batch, int32_t n) { return dense_bat_bytes(dt, batch, n) + dense_spill_bytes(dt, n); }
 
+// The automatic method's choice for a dense call: the batched kernels up to
+// their capacity, except that a call of fewer matrices than the device has
+// CUs (which leaves most CUs idle in the batched kernels) takes the blocked
+// path above PFH_TUNE_SWITCH_N (built-in: PFH_SWITCH_N_DEFAULT).
+#ifndef PFH_SWITCH_N_DEFAULT
+#define PFH_SWITCH_N_DEFAULT 256
+#endif
+bool dense_uses_batched(int dt, int64_t batch, int32_t n) {
+    if (n > reg_max_n(dt)) return false;
+    if (n <= 64 || batch >= device_cu_count()) return true;
+    const int tv = tune_value(PFH_TUNE_SWITCH_N);
+    return n <= (tv >= 64 ? tv : PFH_SWITCH_N_DEFAULT);
+}
+
 size_t slog_ws_bytes(int dt, int64_t batch, int32_t n, int method) {
-    if (n <= reg_max_n(dt) && method == PFH_METHOD_AUTO) return dense_ws_bytes(dt, batch, n);
+    if (method == PFH_METHOD_AUTO && dense_uses_batched(dt, batch, n)) return dense_ws_bytes(dt, batch, n);
     const size_t es = dtype_bytes(dt);
-    if (method == PFH_METHOD_BLOCKED || method == PFH_METHOD_MIXED ||
-        (method == PFH_METHOD_AUTO && n > reg_max_n(dt)))
+    if (method == PFH_METHOD_BLOCKED || method == PFH_METHOD_MIXED || method == PFH_METHOD_AUTO)
         return blocked_workspace_bytes(dt, n, method == PFH_METHOD_MIXED);
     return align_up(es * (size_t)n * n) + align_up(sizeof(LargeState<c128>));
 }
@@ -323,7 +345,7 @@ int slog_dense(int dt, const T* A, int64_t batch, int32_t n, int method, double*
     if (method < PFH_METHOD_AUTO || method > PFH_METHOD_MIXED) return PFH_EINVAL;
     if (batch == 0) return 0;
     if (!A || !log_out || !phase_out) return PFH_EINVAL;
-    if (method == PFH_METHOD_AUTO && n <= reg_max_n(dt)) {
+    if (method == PFH_METHOD_AUTO && dense_uses_batched(dt, batch, n)) {
         BatchDesc d{};
         d.data = A; d.count = batch; d.dense_n = n; d.log_out = log_out; d.phase_out = phase_out;
         const size_t sb = dense_ws_bytes(dt, batch, n);
@@ -354,7 +376,7 @@ int slog_dense(int dt, const T* A, int64_t batch, int32_t n, int method, double*
     int err = resolve_ws(ws_in, ws_bytes, need, &ws);
     if (err) return err;
     const bool blocked = (method != PFH_METHOD_UNBLOCKED);
-    const int nb = blocked_panel_width(dt);
+    const int nb = (method == PFH_METHOD_MIXED && dt == PFH_F64) ? 32 : blocked_panel_width(dt);
     for (int64_t b = 0; b < batch; ++b) {
         const T* Ab = A + b * (int64_t)n * n;
         if (blocked) {
