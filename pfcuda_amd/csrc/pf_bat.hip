@@ -75,10 +75,12 @@ typedef double bat_d4 __attribute__((ext_vector_type(4)));
 
 // Entry (r, c) of a matrix whose upper triangle is stored with row stride
 // ld; the load is unconditional (the diagonal when r == c).
+// (Offsets within one matrix or slab stay below 256^2 elements: 32-bit
+// arithmetic from the wave-uniform base.)
 template <typename T>
 __device__ __forceinline__ T bat_w(const T* W, int ld, int r, int c) {
     const int a = r < c ? r : c, b = r < c ? c : r;
-    const T v = W[(int64_t)a * ld + b];
+    const T v = W[(unsigned)(a * ld + b)];
     return r < c ? v : (r > c ? -v : zero<T>());
 }
 
@@ -88,10 +90,10 @@ __device__ __forceinline__ T bat_sel(bool on, T v) { return on ? v : zero<T>(); 
 // The same entry split in two, so that a run of loads can be issued before
 // any of them is used: the element index, and a sign code from the indices
 // alone (+1 above the diagonal, -1 below, 0 on it).
-__device__ __forceinline__ int64_t bat_ix(int ld, int r, int c, int& sg) {
+__device__ __forceinline__ unsigned bat_ix(int ld, int r, int c, int& sg) {
     sg = r < c ? 1 : (r > c ? -1 : 0);
     const int a = r < c ? r : c, b = r < c ? c : r;
-    return (int64_t)a * ld + b;
+    return (unsigned)(a * ld + b);
 }
 template <typename T>
 __device__ __forceinline__ T bat_apply(int sg, T v) { return sg > 0 ? v : (sg < 0 ? -v : zero<T>()); }
@@ -102,8 +104,9 @@ __device__ __forceinline__ double bat_mag(c128 z) { return z.re * z.re + z.im * 
 
 // workgroup argmax, part 1: the wave's best (|value|, key) into LDS
 __device__ __forceinline__ void bat_wave_best(double best, int key, double* r_best, int* r_key) {
-    const double b = wave_max(best);                                            // inputs >= -1, no NaN
-    const int k = wave_min((best == b && best >= 0.0) ? key : 0x7fffffff);      // ties: lowest key
+    double b;
+    int k;
+    wave_argmax(best, key, b, k);                                               // ties: lowest key
     if ((threadIdx.x & 63) == 0) { r_best[threadIdx.x >> 6] = b; r_key[threadIdx.x >> 6] = k; }
 }
 
@@ -266,12 +269,16 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
             // raw loads now, signs applied where the values are used
             auto gather = [&](int I, int J, T (&w4)[4], int (&sg4)[4]) {
                 const int scc = Qs[min(16 * J + lr, m - 1)] + k0;
-                int64_t ix[4];
+                unsigned ix[4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) ix[e] = bat_ix(ld, Qs[min(16 * I + lq + 4 * e, m - 1)] + k0, scc, sg4[e]);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) w4[e] = src[ix[e]];
             };
+            // full panels (np == NB) need no masks on the K index: the
+            // tile loop is compiled once with them and once without
+            auto trail = [&](auto full_c) {
+            constexpr bool FULL = decltype(full_c)::value;
             int I = 0, J = 0;
             T wv4[4] = {zero<T>(), zero<T>(), zero<T>(), zero<T>()};
             int sg[4] = {0, 0, 0, 0};
@@ -299,7 +306,7 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
                 for (int g = 0; g < (2 * NB + 3) / 4; ++g) {
                     const int q = 4 * g + lq;                        // K index: pair q / 2
                     const int j = q >> 1;
-                    const bool on = j < np;
+                    const bool on = FULL || j < np;
                     const int jj = on ? j : 0;
                     const T a = bat_sel(on, Pp[2 * (jj * NP + sA) + (q & 1)]);         // U_j, V_j at the row
                     const T b0 = Pp[2 * (jj * NP + sB) + ((q & 1) ^ 1)];              // V_j, U_j at the column
@@ -321,7 +328,7 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
                         T v;
                         if constexpr (is_cplx<T>::value) v = T{w.re + acc[e], w.im + aci[e]};
                         else v = w + acc[e];
-                        dst[(int64_t)(lo + rr) * NP + (lo + cc)] = v;
+                        dst[(unsigned)((lo + rr) * NP + (lo + cc))] = v;
                     }
                 }
                 I = In;
@@ -329,6 +336,9 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
 #pragma unroll
                 for (int e = 0; e < 4; ++e) { wv4[e] = wn[e]; sg[e] = sgn[e]; }
             }
+            };
+            if (np == NB) trail(std::true_type{});
+            else trail(std::false_type{});
             src = dst;
             ld = NP;
             dst = (dst == slab0) ? slab1 : slab0;

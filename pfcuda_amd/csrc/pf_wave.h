@@ -49,4 +49,38 @@ __device__ __forceinline__ int wave_min(int k) {
     return __builtin_amdgcn_readlane(k, 63);
 }
 
+// Steps whose DPP pattern gives every lane a value: no "old" operand, so
+// no copy of the source before the move.
+template <int CTRL>
+__device__ __forceinline__ double dpp_all_f64(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// Wave argmax: the largest `best` (inputs >= -1, no NaN) and, among the
+// lanes holding it, the smallest key; (-1, 0x7fffffff) when every lane has
+// -1.  When one lane alone holds the maximum (the usual case) its key is read
+// directly; ties fall back to a min-reduction of the keys, so the result is
+// exactly that of wave_max followed by wave_min.
+__device__ __forceinline__ void wave_argmax(double best, int key, double& out_best, int& out_key) {
+    double m = best;
+    m = fmax(m, dpp_all_f64<0xB1>(m));
+    m = fmax(m, dpp_all_f64<0x4E>(m));
+    m = fmax(m, dpp_all_f64<0x141>(m));
+    m = fmax(m, dpp_all_f64<0x140>(m));
+    m = fmax(m, dpp_f64<0x142, 0xA>(m));
+    m = fmax(m, dpp_f64<0x143, 0xC>(m));
+    const double wb = readlane_f64(m, 63);
+    out_best = wb;
+    const unsigned long long hit = __ballot(best == wb && wb >= 0.0);
+    if (hit == 0) { out_key = 0x7fffffff; return; }
+    if ((hit & (hit - 1)) == 0) {                     // one lane holds the maximum
+        out_key = __builtin_amdgcn_readlane(key, (int)__builtin_ctzll(hit));
+        return;
+    }
+    out_key = wave_min((best == wb) ? key : 0x7fffffff);
+}
+
 }  // namespace pfh
