@@ -196,7 +196,9 @@ int pfh_pfaffian_cpu_f64(double* A, int32_t n, double* out);
 #define PFH_TUNE_BLOCKED_NB_C128 19  /* c128 blocked factorization: 8 or 16 */
 #define PFH_TUNE_SWITCH_N 20       /* dense slog of fewer matrices than the device has CUs: the largest n the
                                       batched kernels take (64, 96, ..., 256); larger ones run blocked */
-#define PFH_TUNE_NKEYS 21
+#define PFH_TUNE_SMALL_KERNEL 21   /* pfaffian / slog, n <= 32: 1 one wave per matrix updating the live rows (the
+                                      default), 0 the column-per-lane kernel that updates the whole square */
+#define PFH_TUNE_NKEYS 22
 
 /* Sets a tuning key (value -1: the built-in choice); PFH_EINVAL for an
  * unknown key or a value outside the key's list.  Takes effect for calls

@@ -41,6 +41,7 @@ CANDIDATES = {
     "bat_nb_c128_128": [4, 6, 8], "bat_nb_c128_192": [4, 6, 8], "bat_nb_c128_256": [6, 8],
     "blocked_nb": [16, 24, 32], "blocked_nb_c128": [8, 16],
     "switch_n": [96, 128, 160, 192, 224, 256],
+    "small_kernel": [0, 1],
 }
 BUCKETS = {"f64_bucket_128": (64, 128), "f64_bucket_192": (128, 192), "f64_bucket_256": (192, 256),
            "c128_bucket_128": (64, 128),
@@ -122,6 +123,19 @@ def _switch_workload(torch, dev):
     return run
 
 
+def _small_workload(torch, dev):
+    """pfaffian() of 2^16 seeded 32 x 32 f64 matrices."""
+    from .api import pfaffian
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(3232)
+    A = _skew(torch, (1 << 16, 32, 32), False, gen, dev)
+
+    def run():
+        p = pfaffian(A)
+        return torch.log(p.abs()), torch.sign(p)
+    return run
+
+
 def _single_workload(torch, name, n, dev):
     """One n x n solve (blocked slog) or gradient, through the host API."""
     from .api import slog_pfaffian
@@ -169,6 +183,8 @@ def tune(out: str | None = None, quick: bool = False, verbose: bool = True, keys
             run = _bucket_workload(torch, name, count, dev)
         elif name == "switch_n":
             run = _switch_workload(torch, dev)
+        elif name == "small_kernel":
+            run = _small_workload(torch, dev)
         else:
             run = _single_workload(torch, name, n_big, dev)
         ref = run()

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpfaffian_hip.so")
 BUILD_DIR = os.path.join(HERE, "_build")
-SOURCES = ["pf_reg.hip", "pf_lds.hip", "pf_bat.hip", "pf_large.hip", "pf_blocked.hip", "pf_capi.hip", "pf_cpu.hip"]
+SOURCES = ["pf_reg.hip", "pf_c32.hip", "pf_lds.hip", "pf_bat.hip", "pf_large.hip", "pf_blocked.hip", "pf_capi.hip", "pf_cpu.hip"]
 ARCH = os.environ.get("PFH_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

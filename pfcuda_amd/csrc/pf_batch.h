@@ -100,6 +100,10 @@ int lds_spill_slabs();           // slabs needed (one per resident workgroup)
 int launch_bat(int dtype, const BatchDesc& d, int maxn, hipStream_t stream);
 int64_t bat_ws_elems(int dtype, int maxn, int64_t count);
 
+// One wave per matrix for n <= 32 (pf_c32.hip): MODE_PROD and MODE_SLOG,
+// every dtype.
+int launch_c32(int dtype, int mode, const BatchDesc& d, hipStream_t stream);
+
 // Launches the workgroup kernel whose capacity covers `maxn` for every
 // matrix in the batch.  Returns 0 or a hipError_t value.
 int launch_reg(int dtype, int mode, const BatchDesc& d, int maxn, hipStream_t stream);

@@ -566,8 +566,9 @@ __device__ __forceinline__ void bc_poll(const unsigned long long* rec_all, int e
         const double best = (m == m) ? m : -1.0;
         const int idx = (int)((wb >> 32) & 0x7fffffffull);
         const int slot = (int)(unsigned)(wb & 0xffffffffull);
-        const double b = wave_max(best);                                       // inputs >= -1, no NaN
-        const int bi = wave_min((best == b && b >= 0.0) ? idx : 0x7fffffff);   // ties: lowest index
+        double b;
+        int bi;
+        wave_argmax(best, idx, b, bi);                                          // ties: lowest index
         const unsigned long long hit = __ballot(b >= 0.0 && best == b && idx == bi);
         const int w = hit ? (int)__builtin_ctzll(hit) : -1;
         if (g == 0) *s_w = w;
@@ -597,8 +598,9 @@ __device__ __forceinline__ void bc_publish(double best, int bi, T val, int lg, i
                                            int* r_idx, int* r_t, int* r_slot) {
     constexpr bool CAND = panel_cand<T>();
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const double b = wave_max(best);                                    // inputs >= -1, no NaN
-    const int ix = wave_min((best == b && best >= 0.0) ? bi : 0x7fffffff);   // ties: lowest index
+    double b;
+    int ix;
+    wave_argmax(best, bi, b, ix);                                       // ties: lowest index
     if (lane == 0) { r_best[wv] = b; r_idx[wv] = ix; }
     __syncthreads();
     double gb = r_best[0];

@@ -155,8 +155,9 @@ pf_reg_kernel(BatchDesc d) {
                     if (m > best || (m == best && key < bk)) { best = m; bk = key; }
                 }
             }
-            const double wb = wave_max(best);                                  // inputs >= -1, no NaN
-            const int wk = wave_min((best == wb && wb >= 0.0) ? bk : 0x7fffffff);
+            double wb;
+            int wk;
+            wave_argmax(best, bk, wb, wk);                                     // inputs >= -1, no NaN
             int lq, q;
             if (wk == 0x7fffffff) {                // only NaN candidates: keep k+1
                 lq = k + 1;
@@ -398,7 +399,23 @@ static int f64_kernel(int mode, int maxn, const BatchDesc& d) {
     return maxn > 64 ? 2 : 0;
 }
 
+// n <= 32, product form and slog: the one-wave live-row kernel
+// (pf_c32.hip) unless PFH_TUNE_SMALL_KERNEL (or PFH_SMALL_KERNEL) is 0,
+// which keeps the column-per-lane kernel above.
+static bool c32_enabled() {
+    const int tv = tune_value(PFH_TUNE_SMALL_KERNEL);
+    if (tv >= 0) return tv == 1;
+    static const int env = [] {
+        const char* e = getenv("PFH_SMALL_KERNEL");
+        return (e && e[0] == '0') ? 0 : 1;
+    }();
+    return env == 1;
+}
+
 int launch_reg(int dtype, int mode, const BatchDesc& d, int maxn, hipStream_t s) {
+    if (maxn <= 32 && (mode == MODE_PROD || mode == MODE_SLOG) && c32_enabled() &&
+        !(dtype == DT_F64 && f64_kernel_override() == 2))
+        return launch_c32(dtype, mode, d, s);
     if (dtype == DT_F64) {
         const int kf = f64_kernel(mode, maxn, d);
         if (kf == 2) return launch_lds_f64(mode, d, maxn, s);

@@ -264,3 +264,61 @@ def test_host_pointer_forms_vs_oracle(pf, cuda, dtype):
         assert abs(complex(ph) - complex(p0)) <= tol
     la, ph = pf.slog_pfaffian_host(np.zeros((7, 7), dtype=dtype))
     assert la == -math.inf and ph == 0
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128, torch.float32, torch.complex64])
+def test_small_kernels_vs_oracle(pf, cuda, dtype):
+    # n <= 32: the one-wave live-row kernel (the default) and the
+    # column-per-lane kernel, product form and slog, every even and odd size,
+    # a zero matrix and one whose first pivot row is zero past its first
+    # entry (an interchange at every step of a permuted block matrix)
+    from pfcuda_amd import _lib
+    cplx = dtype.is_complex
+    lo = dtype in (torch.float32, torch.complex64)
+    rng = np.random.default_rng(3232)
+    key = _lib.TUNE_KEYS["small_kernel"]
+    mats = [oracle.skew_gaussian(rng, n, complex_=cplx) for n in range(1, 33)]
+    mats.append(np.zeros((32, 32), dtype=mats[-1].dtype))
+    P = np.eye(32)[rng.permutation(32)]
+    mats.append(P @ oracle.skew_gaussian(rng, 32, complex_=cplx) @ P.T)
+    sizes = [int(x) for x in rng.integers(0, 33, size=300)]
+    ms = [torch.from_numpy(oracle.skew_gaussian(rng, s, complex_=cplx)).to(dtype) for s in sizes]
+
+    def lo_tol(A):
+        # single precision: the relative error of Pf (the error of log|Pf|)
+        # grows with n and the condition number of the rounded input
+        if A.shape[0] == 0:
+            return 1e-5
+        c = np.linalg.cond(A)
+        return 1e-5 + 1.2e-7 * A.shape[0] * (c if np.isfinite(c) else 1e30)
+
+    try:
+        for v in (1, 0):
+            _lib.check(_lib.lib.pfh_set_tuning(key, v), "small_kernel")
+            for A in mats:
+                n = A.shape[0]
+                At = torch.from_numpy(A).to(dtype)
+                Ar = At.to(torch.complex128 if cplx else torch.float64).numpy()
+                want = oracle.pf_c128(Ar) if cplx else oracle.pf_f64(Ar)
+                got = complex(pf.pfaffian(At.to(cuda)).item()) if n > 4 else complex(want[0])
+                tol = lo_tol(Ar) if lo else 1e-12 * max(1.0, n / 16)
+                assert abs(got - complex(want[0])) <= tol * max(abs(want[0]), 1e-300), (v, n, got, want[0])
+                if n >= 2 and not (n & 1):
+                    gl, gp = pf.slog_pfaffian_batched(At.to(cuda))
+                    if math.isinf(want[1]):
+                        assert math.isinf(float(gl)) and complex(gp.item()) == 0
+                    else:
+                        assert abs(float(gl) - want[1]) <= (tol if lo else 1e-11), (v, n)
+            # a ragged batch of small sizes through the same kernels
+            data, offs, sz = pf.pack_ragged(ms)
+            gl, gp = pf.slog_pfaffian_ragged(data.to(cuda), offs, sz)
+            gl = gl.cpu().numpy()
+            for i, m in enumerate(ms):
+                mr = m.to(torch.complex128 if cplx else torch.float64).numpy()
+                l0, _ = oracle.slog(mr)
+                if math.isinf(l0):
+                    assert math.isinf(gl[i]), (v, sizes[i])
+                else:
+                    assert abs(gl[i] - l0) <= (lo_tol(mr) if lo else 1e-11), (v, sizes[i])
+    finally:
+        _lib.lib.pfh_set_tuning(key, -1)
