@@ -183,7 +183,7 @@ int pfh_pfaffian_cpu_f64(double* A, int32_t n, double* out);
                                        panel sets its error flag (output NaN), 1 .. 4194304 (the default);
                                        0 sets it at once.  For tests of that path, not a tuning choice */
 #define PFH_TUNE_PANEL_XCD 10      /* blocked panel exchange: 1 through one XCD's L2 when the panel's blocks
-                                       fit one XCD, 0 device-wide; built-in: 1 for complex, 0 for real types */
+                                       fit one XCD, 0 device-wide; built-in: 1 */
 #define PFH_TUNE_MIXED_SLICES 11   /* PFH_METHOD_MIXED: 7-bit digits per value of the split-precision trailing
                                        update, 5 .. 9 */
 #define PFH_TUNE_BAT_NB_128 12     /* f64 batched blocked kernel, 64 < n <= 128: pairs per panel, 6, 8 or 12 */

@@ -588,14 +588,81 @@ __device__ __forceinline__ bool bc_better(double b, int i, double bb, int bi) {
     return b > bb || (b == bb && i < bi);
 }
 
+// Own-slot panel vectors in registers.  A vector of more than 16 values with
+// a dynamic index goes to scratch memory, so the NB values sit in two halves
+// of at most 16; a dynamic index is only ever taken within one half, named at
+// compile time (the pair steps run in stages of 16), and compiles to
+// register indexing.  Every dynamic index must lie inside its half even on
+// paths that do not use the result: the compiler may execute an indexed
+// write speculatively (both arms of a branch, then a select), and an index
+// past the vector would overwrite whatever register follows it.  The callers
+// clamp such indices (masking them inside the accessors instead sent the
+// vectors to scratch memory).
+template <typename R, int N>
+struct BPRegHalf {
+    static constexpr int L0 = N > 16 ? 16 : N, L1 = N > 16 ? N - 16 : 8;
+    typedef R V0 __attribute__((ext_vector_type(L0)));
+    typedef R V1 __attribute__((ext_vector_type(L1)));
+    V0 v0;
+    V1 v1;
+    template <int J> __device__ __forceinline__ R at() const {
+        if constexpr (J < 16) return v0[J];
+        else return v1[J - 16];
+    }
+    template <int J> __device__ __forceinline__ void put(R x) {
+        if constexpr (J < 16) v0[J] = x;
+        else v1[J - 16] = x;
+    }
+    template <int H> static constexpr int len() { return H == 0 ? L0 : L1; }
+    template <int H> __device__ __forceinline__ R get(int i) const {
+        if constexpr (H == 0) return v0[i];
+        else return v1[i];
+    }
+    template <int H> __device__ __forceinline__ void set(int i, R x) {
+        if constexpr (H == 0) v0[i] = x;
+        else v1[i] = x;
+    }
+};
+template <typename T, int N>
+struct BPRegs {
+    BPRegHalf<T, N> v;
+    template <int J> __device__ __forceinline__ T at() const { return v.template at<J>(); }
+    template <int J> __device__ __forceinline__ void put(T x) { v.template put<J>(x); }
+    template <int H> __device__ __forceinline__ T get(int i) const { return v.template get<H>(i); }
+    template <int H> __device__ __forceinline__ void set(int i, T x) { v.template set<H>(i, x); }
+};
+template <typename R, int N>
+struct BPRegs<cplx<R>, N> {
+    BPRegHalf<R, N> re, im;
+    template <int J> __device__ __forceinline__ cplx<R> at() const { return {re.template at<J>(), im.template at<J>()}; }
+    template <int J> __device__ __forceinline__ void put(cplx<R> x) { re.template put<J>(x.re); im.template put<J>(x.im); }
+    template <int H> __device__ __forceinline__ cplx<R> get(int i) const {
+        return {re.template get<H>(i), im.template get<H>(i)};
+    }
+    template <int H> __device__ __forceinline__ void set(int i, cplx<R> x) {
+        re.template set<H>(i, x.re);
+        im.template set<H>(i, x.im);
+    }
+};
+
+// f(std::integral_constant<int, I>) for I = I0 .. N-1
+template <int I, int N, typename F>
+__device__ __forceinline__ void bp_sfor(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        bp_sfor<I + 1, N>(f);
+    }
+}
+
 // Block argmax of (best, logical index) and publication of the winner for
 // epoch e: with candidate rows, its entries U_j, V_j (j < cnt) and its value
-// (V_cnt) first; then, once this block's exchange stores have landed, the
-// record.
-template <typename T, int NB, int XM>
-__device__ __forceinline__ void bc_publish(double best, int bi, T val, int lg, int x, bool own, int cnt, int e,
-                                           int rank, const BPCoop<T>& co, const T* Ps, double* r_best,
-                                           int* r_idx, int* r_t, int* r_slot) {
+// V_cnt first (the winning thread passes them from its registers through
+// LDS, Wx, to 2 cnt + 2 storing threads); then, once this block's exchange
+// stores have landed, the record.
+template <typename T, int NB, int XM, typename Regs>
+__device__ __forceinline__ void bc_publish(double best, int bi, int lg, int x, bool own, int cnt, int e, int rank,
+                                           const BPCoop<T>& co, const Regs& Ur, const Regs& Vr, T* Wx,
+                                           double* r_best, int* r_idx, int* r_slot) {
     constexpr bool CAND = panel_cand<T>();
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     double b;
@@ -608,17 +675,24 @@ __device__ __forceinline__ void bc_publish(double best, int bi, T val, int lg, i
     for (int u = 1; u < BC_WAVES; ++u)
         if (bc_better(r_best[u], r_idx[u], gb, gi)) { gb = r_best[u]; gi = r_idx[u]; }
     const bool found = gb >= 0.0;
-    T* cand = co.cand + ((int64_t)(e % 3) * BC_MAXG + rank) * 2 * NB;
     if (found && own && best >= 0.0 && lg == gi) {
         *r_slot = x;
         if constexpr (CAND) {
-            *r_t = t;
-            xb_st<XM>(&cand[2 * cnt + 1], val);
+            // pairs 0 .. cnt in chunks of 8 (the entries past V_cnt are zero)
+            bp_sfor<0, NB / 8>([&](auto cc) {
+                constexpr int C = decltype(cc)::value;
+                if (8 * C > cnt) return;
+                bp_sfor<0, 8>([&](auto qc) {
+                    constexpr int J = 8 * C + decltype(qc)::value;
+                    reinterpret_cast<BCPair<T>*>(Wx)[J] = BCPair<T>{Ur.template at<J>(), Vr.template at<J>()};
+                });
+            });
         }
     }
     if constexpr (CAND) {
         __syncthreads();
-        if (found && t < 2 * cnt) xb_st<XM>(&cand[t], Ps[2 * ((t >> 1) * BC_T + *r_t) + (t & 1)]);
+        T* cand = co.cand + ((int64_t)(e % 3) * BC_MAXG + rank) * 2 * NB;
+        if (found && t < 2 * cnt + 2) xb_st<XM>(&cand[t], Wx[t]);
     }
     if constexpr (XM != XM_FENCE) __builtin_amdgcn_s_waitcnt(0);   // this thread's stores have landed
     else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");        // this thread's stores are visible device-wide
@@ -644,13 +718,18 @@ __device__ __forceinline__ void bc_publish(double best, int bi, T val, int lg, i
 // (XM_XCD).  Were fewer to land there, a rank would stay unclaimed and the
 // poll would time out (NaN, reported by pfh_stream_status), never a wrong
 // value.
+//
+// The own slot's panel entries U_j, V_j live in registers (the corrections
+// read them with static indices, in chunks; only U_i, V_i and V_{i+1} of the
+// current pair step take a dynamic index, within one 16-value half).  In LDS
+// it took ~1.9k of ~10.5k cycles per pair step to read them (N = 8192 f64).
 template <typename T, int NB, bool LOCAL>
 __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int k0, int np, int epoch0) {
     constexpr int XM = panel_xm<T, LOCAL>();
-    extern __shared__ unsigned char smem_raw[];
-    // Panel entries of the own slot, interleaved so that one 16-byte LDS read
-    // returns a pair: U_j at Ps[2*(j*BC_T + t)], V_j at Ps[2*(j*BC_T + t) + 1].
-    T* Ps = reinterpret_cast<T*>(smem_raw);
+    constexpr bool CAND = panel_cand<T>();
+    constexpr int NH = (NB + 15) / 16;                  // stages of 16 pair steps
+    constexpr int CH = is_cplx<T>::value ? 4 : 8;       // pairs per chunk of the corrections
+    static_assert(NB % 8 == 0 && NB <= 32, "panel widths 8, 16, 24, 32");
     __shared__ int Qpan[2 * NB + 2];           // slot of logical k0 + q
     // entries at one slot, interleaved (U_j at 2j, V_j at 2j + 1): the
     // winner's (the new row k+1, its value V_i at 2i + 1), and those at the
@@ -658,8 +737,9 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
     // one of them is an index, not a select between two loads): Bx[0]
     // winner, Bx[1] pre-interchange k+1, Bx[2] k+2
     __shared__ __align__(16) T Bx[3][2 * NB];
+    __shared__ __align__(16) T Wx[2 * NB];     // the block winner's entries (candidate rows)
     __shared__ double r_best[BC_WAVES];
-    __shared__ int r_idx[BC_WAVES], r_t, r_slot;
+    __shared__ int r_idx[BC_WAVES], r_slot;
     __shared__ int s_w, s_p, s_sp, s_fail;
 
     __shared__ int s_rank;
@@ -678,6 +758,7 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
     const int t = threadIdx.x;
     const int x = k0 + rank * BC_T + t;         // own slot
     const bool own = x < n;
+    const int xs = own ? x : k0;                // a valid slot for masked reads
     T* PU = a.P;                                // U_j[x] at PU[j*n + x]
     T* PV = a.P + (int64_t)NB * n;              // V_j[x] at PV[j*n + x]
 
@@ -688,7 +769,12 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
 
     for (int q = t; q < 2 * NB + 2; q += BC_T) Qpan[q] = k0 + q;
     int lg = x;                                 // logical index of the own slot
-    for (int j = 0; j < NB; ++j) { Ps[2 * (j * BC_T + t)] = zero<T>(); Ps[2 * (j * BC_T + t) + 1] = zero<T>(); }
+    BPRegs<T, NB> Ur, Vr;                       // U_j, V_j at the own slot
+    bp_sfor<0, NB>([&](auto jc) {
+        constexpr int J = decltype(jc)::value;
+        Ur.template put<J>(zero<T>());
+        Vr.template put<J>(zero<T>());
+    });
 
     // first pivot row of the panel: row k0 of W (nothing pending); epoch
     // epoch0 + 1 + i carries the candidates of pair step i
@@ -697,143 +783,184 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
         T r = zero<T>();
         if (own && x > k0) {
             r = a.W[(int64_t)k0 * n + x];
-            Ps[2 * t + 1] = r;
             xb_st<XM>(&PV[x], r);
             const double m = (double)mag2(r);
             if (m == m) best = m;
         }
-        bc_publish<T, NB, XM>(best, lg, r, lg, x, own, 0, epoch0 + 1, rank, co, Ps, r_best, r_idx, &r_t, &r_slot);
+        Vr.template put<0>(r);
+        bc_publish<T, NB, XM>(best, lg, lg, x, own, 0, epoch0 + 1, rank, co, Ur, Vr, Wx, r_best, r_idx, &r_slot);
     }
 
-    for (int i = 0; i < np; ++i) {
-        const int k = k0 + 2 * i;
-        const int e = epoch0 + 1 + i;
-        const bool has_next = i + 1 < np;       // row k+2 is this panel's next pivot row
-        const int sa = Qpan[k + 1 - k0];
-        const int sb = has_next ? Qpan[k + 2 - k0] : -1;
-        bc_poll<XM>(co.rec, e, G, a.spin, &st->err, &s_w, &s_p, &s_sp, &s_fail);
-        if (s_fail) return;                                // err is set: the result will be NaN
-        const int wb = s_w;
-        int p, sp;
-        if (wb >= 0) { p = s_p; sp = s_sp; }
-        else { p = k + 1; sp = sa; }                       // no finite candidate (NaN row)
-        const bool sw = p != k + 1;
-        // the interchange k+1 <-> p as seen from the own slot
-        if (x == sp) lg = k + 1;
-        else if (sw && x == sa) lg = p;
-        const int y1 = sp;
-        const int y2 = (p == k + 2) ? sa : sb;
-        const int y2s = has_next ? y2 : y1;
-        const int sbs = sb >= 0 ? sb : sa;
-        const int xs = own ? x : k0;                       // a valid slot for masked reads
+    int quit = 0;   // 1: the panel is done (or the matrix); 2: leave at once (zero pivot, failed poll)
+    bp_sfor<0, NH>([&](auto hc) {
+        constexpr int H = decltype(hc)::value;
+        if (quit) return;
+        const int iend = min(np, 16 * H + 16);
+        for (int i = 16 * H; i < iend; ++i) {
+            const int il = i - 16 * H;
+            const int k = k0 + 2 * i;
+            const int e = epoch0 + 1 + i;
+            const bool has_next = i + 1 < np;       // row k+2 is this panel's next pivot row
+            const int sa = Qpan[k + 1 - k0];
+            const int sb = has_next ? Qpan[k + 2 - k0] : -1;
+            const int sbs = sb >= 0 ? sb : sa;
 
-        // ---- one round of loads (each unconditional from a valid address,
-        //      masked after): the winner's entries and value, the entries at
-        //      the pre-interchange slots of rows k+1 and k+2, and rows k+1
-        //      and k+2 of W at the own slots
-        if (t < 2 * NB) {                                  // the lanes that keep them
-            const int j = t >> 1;
-            const bool isv = t & 1;
-            const bool ok = j < i || (isv && j == i);
-            const T* src = (isv ? PV : PU) + (int64_t)j * n;
-            const T* w0 = panel_cand<T>()
-                ? co.cand + ((int64_t)(e % 3) * BC_MAXG + (wb >= 0 ? wb : 0)) * 2 * NB + t
-                : &src[sp];
-            const T v0 = xb_ld<XM>(w0), v1 = xb_ld<XM>(&src[sa]), v2 = xb_ld<XM>(&src[sbs]);
-            Bx[0][t] = psel(ok, v0, zero<T>());
-            Bx[1][t] = psel(ok, v1, zero<T>());
-            Bx[2][t] = psel(ok && sb >= 0, v2, zero<T>());
-        }
-        const bool act1 = own && lg >= k + 2;
-        T w1 = wsl(a.W, n, y1, xs);
-        if (!act1) w1 = zero<T>();
-        T w12 = wsl(a.W, n, y1, y2s);
-        if (!has_next) w12 = zero<T>();
-        T w2 = wsl(a.W, n, has_next ? y2 : y1, xs);
-        if (!(has_next && own && lg >= k + 3)) w2 = zero<T>();
-        __syncthreads();
-        const T* b1 = Bx[(wb >= 0) ? 0 : 1];               // the pivot's entries (row k+1 at the start)
-        const T piv = b1[2 * i + 1];                       // V_i there
-        if (is_zero(piv)) {                                // same decision in every block
-            if (keeper) st->dead = 1;
-            return;
-        }
-        if (keeper) accumulate(lm, le, ph_re, ph_im, piv, sw);
-        // the interchange in the shared map (every read of Qpan in this step
-        // came before the block barrier in bc_poll)
-        if (sw && t == 0) {
-            Qpan[k + 1 - k0] = sp;
-            if ((unsigned)(p - k0) < (unsigned)(2 * NB + 2)) Qpan[p - k0] = sa;
-        }
-        const T* b2 = Bx[(p == k + 2) ? 1 : 2];
-        if (k + 2 >= n) break;                             // last pair of the matrix
-
-        const T inv = recip(piv);
-        // corrections of the pairs j < i, in chunks of eight (entries past i
-        // are zero in b1, b2 and Ps, so a partial chunk adds nothing)
-        // (real types: the own slot's eight pairs are read first, each as
-        // one aligned pair, so the LDS reads go out back to back from one
-        // base address; complex pairs are read as they are used: four read
-        // ahead took N = 4096 from 15.3 to 18.3 ms)
-        // (complex: each product accumulates as four FMAs into separate sums
-        // of the added and the subtracted terms, which halves the dependent
-        // chains; a complex product written out costs six operations)
-        T r1 = zero<T>(), rl = zero<T>(), rk = zero<T>();
-        T r1m = zero<T>(), rlm = zero<T>(), rkm = zero<T>();
-        const BCPair<T>* Pq = reinterpret_cast<const BCPair<T>*>(Ps);
-        for (int j0 = 0; j0 < i; j0 += 8) {
-            constexpr int PRE = is_cplx<T>::value ? 1 : 8;
-            BCPair<T> pv[PRE];
-            if constexpr (PRE == 8) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) pv[q] = Pq[(j0 + q) * BC_T + t];
+            // ---- loads that do not depend on the pivot, issued before the
+            //      poll: the entries at the pre-interchange slots of rows k+1
+            //      and k+2 that this epoch's records do not cover (U_j for
+            //      j < i-1, V_j for j < i: stored before an earlier record;
+            //      U_{i-1} and V_i come after the poll), by wave 1, so that
+            //      the polling wave does not wait for them; and row k+2 of W
+            //      at the own slots (its slot changes only when the pivot is
+            //      k+2)
+            if (t >= 64 && t < 64 + 2 * NB) {
+                const int l = t - 64, j = l >> 1;
+                const bool old = (l & 1) ? j < i : j < i - 1;
+                const T* src = ((l & 1) ? PV : PU) + (int64_t)j * n;
+                const T v1 = xb_ld<XM>(&src[sa]), v2 = xb_ld<XM>(&src[sbs]);
+                Bx[1][l] = psel(old, v1, zero<T>());
+                Bx[2][l] = psel(old && sb >= 0, v2, zero<T>());
             }
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int j = j0 + q;
-                T uj, vj;
-                if constexpr (PRE == 8) { uj = pv[q].u; vj = pv[q].v; }
-                else { uj = Ps[2 * (j * BC_T + t)]; vj = Ps[2 * (j * BC_T + t) + 1]; }
-                const T b1u = b1[2 * j], b1v = b1[2 * j + 1];
-                const T b2u = b2[2 * j], b2v = b2[2 * j + 1];
-                if constexpr (is_cplx<T>::value) {
-                    cfma(r1, b1u, vj); cfma(r1m, b1v, uj);
-                    cfma(rl, b2u, vj); cfma(rlm, b2v, uj);
-                    cfma(rk, b1u, b2v); cfma(rkm, b1v, b2u);
-                } else {
-                    r1 = r1 + b1u * vj - b1v * uj;
-                    rl = rl + b2u * vj - b2v * uj;
-                    rk = rk + b1u * b2v - b1v * b2u;
+            T w2 = zero<T>();
+            if (has_next) w2 = wsl(a.W, n, sb, xs);
+
+            bc_poll<XM>(co.rec, e, G, a.spin, &st->err, &s_w, &s_p, &s_sp, &s_fail);
+            if (s_fail) { quit = 2; return; }                  // err is set: the result will be NaN
+            const int wb = s_w;
+            int p, sp;
+            if (wb >= 0) { p = s_p; sp = s_sp; }
+            else { p = k + 1; sp = sa; }                       // no finite candidate (NaN row)
+            const bool sw = p != k + 1;
+            // the interchange k+1 <-> p as seen from the own slot
+            if (x == sp) lg = k + 1;
+            else if (sw && x == sa) lg = p;
+            const int y1 = sp;
+            const int y2 = (p == k + 2) ? sa : sb;
+            const int y2s = has_next ? y2 : y1;
+
+            // ---- the loads that do (each unconditional from a valid address,
+            //      masked after): the winner's entries and value, U_{i-1}
+            //      and V_i at the pre-interchange slots of rows k+1 and k+2,
+            //      and rows k+1 (and k+2 when the pivot is k+2) of W at the
+            //      own slots
+            if (t < 2 * NB) {
+                const int j = t >> 1;
+                const bool isv = t & 1;
+                const bool ok = j < i || (isv && j == i);
+                const T* src = (isv ? PV : PU) + (int64_t)j * n;
+                const T* w0 = CAND ? co.cand + ((int64_t)(e % 3) * BC_MAXG + (wb >= 0 ? wb : 0)) * 2 * NB + t
+                                   : &src[sp];
+                const T v0 = xb_ld<XM>(w0);
+                Bx[0][t] = psel(ok, v0, zero<T>());
+                if (isv ? j == i : j == i - 1) {
+                    const T v1 = xb_ld<XM>(&src[sa]), v2 = xb_ld<XM>(&src[sbs]);
+                    Bx[1][t] = v1;
+                    Bx[2][t] = psel(sb >= 0, v2, zero<T>());
                 }
             }
-        }
-        if constexpr (is_cplx<T>::value) {
-            r1 = r1 - r1m;
-            rl = rl - rlm;
-            rk = rk - rkm;
-        }
-        T u = zero<T>();
-        if (act1) {
-            u = (w1 + r1) * inv;
-            xb_st<XM>(&PU[(int64_t)i * n + x], u);
-        }
-        Ps[2 * (i * BC_T + t)] = u;
-        if (!has_next) break;
+            const bool act1 = own && lg >= k + 2;
+            T w1 = wsl(a.W, n, y1, xs);
+            if (!act1) w1 = zero<T>();
+            T w12 = wsl(a.W, n, y1, y2s);
+            if (!has_next) w12 = zero<T>();
+            if (has_next && p == k + 2) w2 = wsl(a.W, n, sa, xs);
+            if (!(has_next && own && lg >= k + 3)) w2 = zero<T>();
+            __syncthreads();
+            const T* b1 = Bx[(wb >= 0) ? 0 : 1];               // the pivot's entries (row k+1 at the start)
+            const T piv = b1[2 * i + 1];                       // V_i there
+            if (is_zero(piv)) {                                // same decision in every block
+                if (keeper) st->dead = 1;
+                quit = 2;
+                return;
+            }
+            if (keeper) accumulate(lm, le, ph_re, ph_im, piv, sw);
+            // the interchange in the shared map (every read of Qpan in this
+            // step came before the block barrier in bc_poll)
+            if (sw && t == 0) {
+                Qpan[k + 1 - k0] = sp;
+                if ((unsigned)(p - k0) < (unsigned)(2 * NB + 2)) Qpan[p - k0] = sa;
+            }
+            const T* b2 = Bx[(p == k + 2) ? 1 : 2];
+            if (k + 2 >= n) { quit = 1; return; }              // last pair of the matrix
 
-        // ---- row k+2 (the next pivot row) at the own slots -> V_{i+1}
-        const T u_kl = (w12 + rk) * inv;   // U_i at row k+2
-        const T v_kl = b2[2 * i + 1];      // V_i at row k+2
-        double best = -1.0;
-        T r = zero<T>();
-        if (own && lg >= k + 3) {
-            r = w2 + rl + u_kl * Ps[2 * (i * BC_T + t) + 1] - v_kl * u;
-            xb_st<XM>(&PV[(int64_t)(i + 1) * n + x], r);
-            const double m = (double)mag2(r);
-            if (m == m) best = m;
+            const T inv = recip(piv);
+            // corrections of the pairs j < i, in chunks (entries past i are
+            // zero in b1, b2 and the registers -- V_i aside, whose
+            // coefficients b1u_i and b2u_i are zero -- so a partial chunk adds
+            // nothing); the chunk's b1, b2 pairs are read first
+            // (complex: each product accumulates as four FMAs into separate
+            // sums of the added and the subtracted terms, which halves the
+            // dependent chains; a complex product written out costs six
+            // operations)
+            T r1 = zero<T>(), rl = zero<T>(), rk = zero<T>();
+            T r1m = zero<T>(), rlm = zero<T>(), rkm = zero<T>();
+            const BCPair<T>* P1 = reinterpret_cast<const BCPair<T>*>(b1);
+            const BCPair<T>* P2 = reinterpret_cast<const BCPair<T>*>(b2);
+            bp_sfor<0, NB / CH>([&](auto cc) {
+                constexpr int C = decltype(cc)::value;
+                if (C * CH >= i) return;
+                BCPair<T> c1[CH], c2[CH];
+                bp_sfor<0, CH>([&](auto qc) {
+                    constexpr int Q = decltype(qc)::value;
+                    c1[Q] = P1[C * CH + Q];
+                    c2[Q] = P2[C * CH + Q];
+                });
+                bp_sfor<0, CH>([&](auto qc) {
+                    constexpr int Q = decltype(qc)::value;
+                    constexpr int J = C * CH + Q;
+                    const T uj = Ur.template at<J>(), vj = Vr.template at<J>();
+                    const T b1u = c1[Q].u, b1v = c1[Q].v, b2u = c2[Q].u, b2v = c2[Q].v;
+                    if constexpr (is_cplx<T>::value) {
+                        cfma(r1, b1u, vj); cfma(r1m, b1v, uj);
+                        cfma(rl, b2u, vj); cfma(rlm, b2v, uj);
+                        cfma(rk, b1u, b2v); cfma(rkm, b1v, b2u);
+                    } else {
+                        r1 = r1 + b1u * vj - b1v * uj;
+                        rl = rl + b2u * vj - b2v * uj;
+                        rk = rk + b1u * b2v - b1v * b2u;
+                    }
+                });
+            });
+            if constexpr (is_cplx<T>::value) {
+                r1 = r1 - r1m;
+                rl = rl - rlm;
+                rk = rk - rkm;
+            }
+            T u = zero<T>();
+            if (act1) {
+                u = (w1 + r1) * inv;
+                xb_st<XM>(&PU[(int64_t)i * n + x], u);
+            }
+            Ur.template set<H>(il, u);
+            if (!has_next) { quit = 1; return; }
+
+            // ---- row k+2 (the next pivot row) at the own slots -> V_{i+1}
+            const T u_kl = (w12 + rk) * inv;   // U_i at row k+2
+            const T v_kl = b2[2 * i + 1];      // V_i at row k+2
+            const T vi = Vr.template get<H>(il);
+            double best = -1.0;
+            T r = zero<T>();
+            if (own && lg >= k + 3) {
+                r = w2 + rl + u_kl * vi - v_kl * u;
+                xb_st<XM>(&PV[(int64_t)(i + 1) * n + x], r);
+                const double m = (double)mag2(r);
+                if (m == m) best = m;
+            }
+            // V_{i+1}: the next half's first entry after the 16th step of a
+            // stage; the clamps only keep a speculated write inside the half
+            // (il + 1 is in range wherever the write is used)
+            if constexpr (H + 1 < NH) {
+                if (il == 15) Vr.template put<16 * (H + 1)>(r);
+                else Vr.template set<H>(min(il + 1, 15), r);
+            } else {
+                constexpr int LH = BPRegHalf<double, NB>::template len<H>();
+                Vr.template set<H>(min(il + 1, LH - 1), r);    // i + 1 < np <= NB
+            }
+            bc_publish<T, NB, XM>(best, lg, lg, x, own, i + 1, e + 1, rank, co, Ur, Vr, Wx, r_best, r_idx,
+                                  &r_slot);
         }
-        Ps[2 * ((i + 1) * BC_T + t) + 1] = r;
-        bc_publish<T, NB, XM>(best, lg, r, lg, x, own, i + 1, e + 1, rank, co, Ps, r_best, r_idx, &r_t, &r_slot);
-    }
+    });
+    if (quit == 2) return;
 
     if (own) a.perm[lg] = x;   // logical -> slot, for the trailing update
     if (keeper) { st->lm = lm; st->le = le; st->ph_re = ph_re; st->ph_im = ph_im; }
@@ -1395,19 +1522,17 @@ static bool coop_enabled() {
 }
 
 // Largest panel grid that the XCD-local form takes: one XCD's CUs (the
-// blocks hold one CU each for their LDS), 0 when the form is off.  By
-// measurement it is on for complex types and off for real ones (ms, XCD-local
-// / device-wide: c128 N = 4096 13.4 / 15.7; f64 N = 4096 12.1 / 9.9,
-// N = 8192 30.5 / 27.0, gradient N = 2048 7.2 / 5.9).  Complex types gain
-// because the device-wide form exchanges them with L2 write-back and
-// invalidate fences every pair step; real types already exchange through
-// agent-scope atomics without fences, and lose because the panel's strided
-// reads of W's columns (one cache line per entry) then all go through one
-// XCD's L2 instead of eight.  PFH_PANEL_XCD=0/1 or PFH_TUNE_PANEL_XCD set
-// it for every type.  Read per call.
-static int panel_xcd_max_grid(bool cplx) {
+// blocks take a CU each), 0 when the form is off.  On by default for every
+// type.  Measured with the own-slot panel vectors in registers (ms,
+// XCD-local / device-wide): f64 N = 4096 9.20 / 11.28, N = 8192 24.5 /
+// 28.6, gradient N = 2048 5.82 / 6.59; c128 N = 4096 12.4 / 14.4.  (With
+// the panel vectors in LDS, real types had lost: f64 N = 4096 12.1 / 9.9,
+// the strided reads of W's columns, one cache line per entry, all going
+// through one XCD's L2.)  PFH_PANEL_XCD=0/1 or PFH_TUNE_PANEL_XCD set it
+// for every type.  Read per call.
+static int panel_xcd_max_grid() {
     const int tv = tune_value(PFH_TUNE_PANEL_XCD);
-    bool on = cplx;
+    bool on = true;
     if (tv >= 0) {
         on = tv == 1;
     } else {
@@ -1422,32 +1547,23 @@ static int panel_xcd_max_grid(bool cplx) {
 template <typename T, int NB, bool LOCAL>
 static hipError_t launch_panel_form(const BPArgs<T>& a, const BPCoop<T>& co, int k0, int np, int epoch0, int G,
                                     hipStream_t s) {
-    const size_t smem = sizeof(T) * 2 * NB * BC_T;
-    static DevCache attr;   // the LDS limit is set on the device
-    hipError_t err = hipSuccess;
-    if (!attr.get([&] {
-            err = hipFuncSetAttribute((const void*)bp_panel<T, NB, LOCAL>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-            return err == hipSuccess ? 1 : 0;
-        }))
-        return err;
-    // A plain launch: the G working blocks, at one per CU (their LDS), are
-    // resident together once the stream's earlier work has drained.  A
-    // block that cannot become resident (the CUs held by other work for
-    // seconds) ends in the barrier's bounded spin, which sets the error
-    // flag: the result is NaN and pfh_stream_status reports PFH_ETIMEOUT;
-    // nothing hangs.  (A cooperative launch, which asks the runtime for
-    // co-residency, was measured slower, 13.5-20.5 ms against 12.6 ms at
-    // N = 4096, and was removed.)
+    // A plain launch: the G working blocks (at most 64; their registers
+    // allow two per CU) are resident together once the stream's earlier work
+    // has drained.  A block that cannot become resident (the CUs held by
+    // other work for seconds) ends in the barrier's bounded spin, which sets
+    // the error flag: the result is NaN and pfh_stream_status reports
+    // PFH_ETIMEOUT; nothing hangs.  (A cooperative launch, which asks the
+    // runtime for co-residency, was measured slower, 13.5-20.5 ms against
+    // 12.6 ms at N = 4096, and was removed.)
     const int grid = LOCAL ? 8 * G : G;
-    hipLaunchKernelGGL((bp_panel<T, NB, LOCAL>), dim3(grid), dim3(BC_T), smem, s, a, co, k0, np, epoch0);
+    hipLaunchKernelGGL((bp_panel<T, NB, LOCAL>), dim3(grid), dim3(BC_T), 0, s, a, co, k0, np, epoch0);
     return hipGetLastError();
 }
 
 template <typename T, int NB>
 static hipError_t launch_panel(const BPArgs<T>& a, const BPCoop<T>& co, int k0, int np, int epoch0, int G,
                                hipStream_t s) {
-    if (G <= panel_xcd_max_grid(is_cplx<T>::value)) return launch_panel_form<T, NB, true>(a, co, k0, np, epoch0, G, s);
+    if (G <= panel_xcd_max_grid()) return launch_panel_form<T, NB, true>(a, co, k0, np, epoch0, G, s);
     return launch_panel_form<T, NB, false>(a, co, k0, np, epoch0, G, s);
 }
 

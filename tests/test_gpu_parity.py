@@ -542,6 +542,37 @@ def test_trailing_forms_bitwise_equal(pf, cuda, monkeypatch, dtype):
     assert abs(res["1"][0] - l0) < 1e-9 and abs(res["1"][1] - p0) < 1e-9
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.complex128, torch.float32])
+def test_panel_exchange_forms_bitwise_equal(pf, cuda, dtype):
+    # The panel's blocks exchange candidates through one XCD's L2 (the
+    # default while the panel fits one XCD) or device-wide; the arithmetic
+    # and its order are the same, so the two agree bit for bit (blocked
+    # factorization at n = 1000, whose panels cross several blocks, and the
+    # blocked sweep at n = 600).
+    from pfcuda_amd import _lib
+    from pfcuda_amd.adjoint import slog_grad
+    rng = np.random.default_rng(62)
+    A = oracle.skew_gaussian(rng, 1000, complex_=dtype.is_complex)
+    B = oracle.skew_gaussian(rng, 600, complex_=dtype.is_complex)
+    At, Bt = torch.from_numpy(A).to(cuda).to(dtype), torch.from_numpy(B).to(cuda).to(dtype)
+    key = _lib.TUNE_KEYS["panel_xcd"]
+    res = {}
+    try:
+        for form in (1, 0):
+            _lib.check(_lib.lib.pfh_set_tuning(key, form), "panel_xcd")
+            la, s = pf.slog_pfaffian(At, method=pf.PFH_METHOD_BLOCKED)
+            G = slog_grad(Bt) if dtype != torch.float32 else torch.zeros(1)
+            res[form] = (float(la), complex(s.item()), G.cpu().numpy())
+        pf.check_status()
+    finally:
+        _lib.lib.pfh_set_tuning(key, -1)
+    assert res[1][0] == res[0][0] and res[1][1] == res[0][1]
+    assert np.array_equal(res[1][2], res[0][2])
+    if dtype != torch.float32:
+        l0, p0 = oracle.slog(A)
+        assert abs(res[1][0] - l0) < 1e-9 and abs(res[1][1] - p0) < 1e-9
+
+
 def test_grad_n2048_residual(pf, cuda):
     # C5 at full size: A G^T = I/2 (G = 1/2 A^-T), and the slog against the oracle
     from pfcuda_amd.adjoint import slog_grad
