@@ -523,8 +523,16 @@ __device__ __forceinline__ unsigned long long rec_ld(const unsigned long long* p
 }
 template <typename T, bool LOCAL>
 constexpr int panel_xm() { return LOCAL ? XM_XCD : (is_cplx<T>::value ? XM_FENCE : XM_AGENT); }
-template <typename T>
-constexpr bool panel_cand() { return !is_cplx<T>::value; }   // the winner's entries through a candidate row
+// The winner's entries reach the other blocks through a candidate row that
+// the winning block writes before its record (real types, device-wide
+// exchange), else they are gathered from the panel vectors at its slot.  In
+// the XCD-local form the gather is the faster (f64 N = 4096 9.26 -> 7.93 ms,
+// N = 8192 24.5 -> 21.9 ms, gradient N = 2048 5.83 -> 5.14 ms): the export
+// of the winner's registers, the row's stores and the wait for them took
+// ~1.3k cycles of each pair step's publication, and reads served by one
+// XCD's L2 gather quickly.
+template <typename T, int XM>
+constexpr bool panel_cand() { return !is_cplx<T>::value && XM != XM_XCD; }
 
 // Wait for every block's record of epoch E and reduce them (one wave, one
 // block per lane): the winning block, or -1 when no block has a finite
@@ -663,7 +671,7 @@ template <typename T, int NB, int XM, typename Regs>
 __device__ __forceinline__ void bc_publish(double best, int bi, int lg, int x, bool own, int cnt, int e, int rank,
                                            const BPCoop<T>& co, const Regs& Ur, const Regs& Vr, T* Wx,
                                            double* r_best, int* r_idx, int* r_slot) {
-    constexpr bool CAND = panel_cand<T>();
+    constexpr bool CAND = panel_cand<T, XM>();
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     double b;
     int ix;
@@ -726,7 +734,7 @@ __device__ __forceinline__ void bc_publish(double best, int bi, int lg, int x, b
 template <typename T, int NB, bool LOCAL>
 __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int k0, int np, int epoch0) {
     constexpr int XM = panel_xm<T, LOCAL>();
-    constexpr bool CAND = panel_cand<T>();
+    constexpr bool CAND = panel_cand<T, XM>();
     constexpr int NH = (NB + 15) / 16;                  // stages of 16 pair steps
     constexpr int CH = is_cplx<T>::value ? 4 : 8;       // pairs per chunk of the corrections
     static_assert(NB % 8 == 0 && NB <= 32, "panel widths 8, 16, 24, 32");
