@@ -56,12 +56,13 @@
 // complex128: twice the LDS per pair.  Measured (ms per bucket, nb 4 / 5-6
 // / 8 / 12): 64-128 0.96 / 0.83 (6) / 1.11 / 1.20; 128-192 3.88 / 2.74 (6) /
 // 2.80; 192-256 7.75 / 7.04 (5) / 5.89 (8); nb = 8 is the widest that keeps
-// two workgroups of NP = 256 on a CU.
+// two workgroups of NP = 256 on a CU.  The autotuner's 2048-matrix buckets
+// later put 128-192 at 8 (ms, nb 4 / 6 / 8: 1.77 / 1.47 / 1.24).
 #ifndef PFH_BAT_NB_C128_128
 #define PFH_BAT_NB_C128_128 6
 #endif
 #ifndef PFH_BAT_NB_C128_192
-#define PFH_BAT_NB_C128_192 6
+#define PFH_BAT_NB_C128_192 8
 #endif
 #ifndef PFH_BAT_NB_C128_256
 #define PFH_BAT_NB_C128_256 8
