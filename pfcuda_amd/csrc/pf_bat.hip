@@ -99,6 +99,24 @@ __device__ __forceinline__ unsigned bat_ix(int ld, int r, int c, int& sg) {
 template <typename T>
 __device__ __forceinline__ T bat_apply(int sg, T v) { return sg > 0 ? v : (sg < 0 ? -v : zero<T>()); }
 
+// Trailing tiles in flight per wave (the ring of the kernel's trailing
+// loop).  Measured on the 16384-matrix batch (ms, f64 / c128): 2 tiles 3.76-
+// 3.82 / 8.89, 3 tiles 3.78-3.83 / 8.91-8.92, 4 tiles 3.90-3.96 / 9.65-9.67;
+// the earlier loop, which gathered the next tile before the current one's
+// products, 3.88 / 9.09.
+#ifndef BAT_RING
+#define BAT_RING 2
+#endif
+
+// f(std::integral_constant<int, I>) for I = I0 .. N-1
+template <int I, int N, typename F>
+__device__ __forceinline__ void bat_sfor(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        bat_sfor<I + 1, N>(f);
+    }
+}
+
 // pivot-search magnitude: |x| for real, |z|^2 for complex (NaN stays NaN)
 __device__ __forceinline__ double bat_mag(double x) { return fabs(x); }
 __device__ __forceinline__ double bat_mag(c128 z) { return z.re * z.re + z.im * z.im; }
@@ -259,8 +277,6 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
             const int m = n - lo;
             const int mt = (m + 15) >> 4;
             const int ntile = mt * (mt + 1) / 2;
-            // software-pipelined over this wave's tiles: the next tile's
-            // stale-entry gathers are issued before this tile's products
             auto tile_ij = [&](int tt, int& I, int& J) {
                 int rem = tt;
                 I = 0;
@@ -277,65 +293,74 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
                 for (int e = 0; e < 4; ++e) w4[e] = src[ix[e]];
             };
             // full panels (np == NB) need no masks on the K index: the
-            // tile loop is compiled once with them and once without
+            // tile loop is compiled once with them and once without.  Each
+            // wave keeps a ring of BAT_RING tiles in flight: a slot's next
+            // tile is gathered right after the slot's products, so the
+            // other slots' products cover the gather.
             auto trail = [&](auto full_c) {
             constexpr bool FULL = decltype(full_c)::value;
-            int I = 0, J = 0;
-            T wv4[4] = {zero<T>(), zero<T>(), zero<T>(), zero<T>()};
-            int sg[4] = {0, 0, 0, 0};
-            if (wv < ntile) {
-                tile_ij(wv, I, J);
-                gather(I, J, wv4, sg);
-            }
-            for (int tt = wv; tt < ntile; tt += NW) {
-                int In = I, Jn = J;
-                T wn[4] = {zero<T>(), zero<T>(), zero<T>(), zero<T>()};
-                int sgn[4] = {0, 0, 0, 0};
-                if (tt + NW < ntile) {
-                    tile_ij(tt + NW, In, Jn);
-                    gather(In, Jn, wn, sgn);
-                }
-                const int cc = 16 * J + lr;
-                const int sA = Qs[min(16 * I + lr, m - 1)], sB = Qs[min(cc, m - 1)];
-                bat_d4 acc = {0.0, 0.0, 0.0, 0.0};
-                bat_d4 aci = {0.0, 0.0, 0.0, 0.0};     // complex: the imaginary part
-                // unrolled to the panel width so that every operand read is
-                // issued ahead of the products (pairs past np are masked);
-                // complex products as four real ones:
-                //   Re += Re a Re b - Im a Im b,  Im += Re a Im b + Im a Re b
+            int TI[BAT_RING], TJ[BAT_RING];
+            T W4[BAT_RING][4];
+            int SG[BAT_RING][4];
+            bat_sfor<0, BAT_RING>([&](auto qc) {
+                constexpr int Q = decltype(qc)::value;
+                TI[Q] = 0; TJ[Q] = 0;
 #pragma unroll
-                for (int g = 0; g < (2 * NB + 3) / 4; ++g) {
-                    const int q = 4 * g + lq;                        // K index: pair q / 2
-                    const int j = q >> 1;
-                    const bool on = FULL || j < np;
-                    const int jj = on ? j : 0;
-                    const T a = bat_sel(on, Pp[2 * (jj * NP + sA) + (q & 1)]);         // U_j, V_j at the row
-                    const T b0 = Pp[2 * (jj * NP + sB) + ((q & 1) ^ 1)];              // V_j, U_j at the column
-                    const T b = bat_sel(on, (q & 1) ? -b0 : b0);
-                    if constexpr (is_cplx<T>::value) {
-                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.re, acc, 0, 0, 0);
-                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, b.im, acc, 0, 0, 0);
-                        aci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.im, aci, 0, 0, 0);
-                        aci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b.re, aci, 0, 0, 0);
-                    } else {
-                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+                for (int e = 0; e < 4; ++e) { W4[Q][e] = zero<T>(); SG[Q][e] = 0; }
+                if (wv + Q * NW < ntile) {
+                    tile_ij(wv + Q * NW, TI[Q], TJ[Q]);
+                    gather(TI[Q], TJ[Q], W4[Q], SG[Q]);
+                }
+            });
+            for (int base = wv; base < ntile; base += BAT_RING * NW) {
+                bat_sfor<0, BAT_RING>([&](auto qc) {
+                    constexpr int Q = decltype(qc)::value;
+                    const int tt = base + Q * NW;
+                    if (tt >= ntile) return;
+                    const int I = TI[Q], J = TJ[Q];
+                    const int cc = 16 * J + lr;
+                    const int sA = Qs[min(16 * I + lr, m - 1)], sB = Qs[min(cc, m - 1)];
+                    bat_d4 acc = {0.0, 0.0, 0.0, 0.0};
+                    bat_d4 aci = {0.0, 0.0, 0.0, 0.0};     // complex: the imaginary part
+                    // unrolled to the panel width so that every operand read is
+                    // issued ahead of the products (pairs past np are masked);
+                    // complex products as four real ones:
+                    //   Re += Re a Re b - Im a Im b,  Im += Re a Im b + Im a Re b
+#pragma unroll
+                    for (int g = 0; g < (2 * NB + 3) / 4; ++g) {
+                        const int q = 4 * g + lq;                        // K index: pair q / 2
+                        const int j = q >> 1;
+                        const bool on = FULL || j < np;
+                        const int jj = on ? j : 0;
+                        const T a = bat_sel(on, Pp[2 * (jj * NP + sA) + (q & 1)]);         // U_j, V_j at the row
+                        const T b0 = Pp[2 * (jj * NP + sB) + ((q & 1) ^ 1)];              // V_j, U_j at the column
+                        const T b = bat_sel(on, (q & 1) ? -b0 : b0);
+                        if constexpr (is_cplx<T>::value) {
+                            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.re, acc, 0, 0, 0);
+                            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, b.im, acc, 0, 0, 0);
+                            aci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.im, aci, 0, 0, 0);
+                            aci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b.re, aci, 0, 0, 0);
+                        } else {
+                            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+                        }
                     }
-                }
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int rr = 16 * I + lq + 4 * e;
-                    if (rr < m && cc < m && cc > rr) {
-                        const T w = bat_apply(sg[e], wv4[e]);
-                        T v;
-                        if constexpr (is_cplx<T>::value) v = T{w.re + acc[e], w.im + aci[e]};
-                        else v = w + acc[e];
-                        dst[(unsigned)((lo + rr) * NP + (lo + cc))] = v;
+                    for (int e = 0; e < 4; ++e) {
+                        const int rr = 16 * I + lq + 4 * e;
+                        if (rr < m && cc < m && cc > rr) {
+                            const T w = bat_apply(SG[Q][e], W4[Q][e]);
+                            T v;
+                            if constexpr (is_cplx<T>::value) v = T{w.re + acc[e], w.im + aci[e]};
+                            else v = w + acc[e];
+                            dst[(unsigned)((lo + rr) * NP + (lo + cc))] = v;
+                        }
                     }
-                }
-                I = In;
-                J = Jn;
-#pragma unroll
-                for (int e = 0; e < 4; ++e) { wv4[e] = wn[e]; sg[e] = sgn[e]; }
+                    const int tn = tt + BAT_RING * NW;             // this slot's next tile
+                    if (tn < ntile) {
+                        tile_ij(tn, TI[Q], TJ[Q]);
+                        gather(TI[Q], TJ[Q], W4[Q], SG[Q]);
+                    }
+                });
             }
             };
             if (np == NB) trail(std::true_type{});
