@@ -99,6 +99,26 @@ __device__ __forceinline__ unsigned bat_ix(int ld, int r, int c, int& sg) {
 template <typename T>
 __device__ __forceinline__ T bat_apply(int sg, T v) { return sg > 0 ? v : (sg < 0 ? -v : zero<T>()); }
 
+// Pairs per panel.  The LDS holds NB NP pairs of panel values (the
+// instantiation's width NB over NP slots); a panel of live width w = n - k0
+// stores its vectors with slot stride w and takes min(cap, NB NP / w) pairs,
+// so the panels widen as the matrix shrinks (and a matrix narrower than NP
+// starts wider): fewer trailing passes at the same occupancy, against longer
+// chains of corrections in the pair steps, hence the cap.  Measured on the
+// 16384-matrix batch (ms, f64 buckets 64-128 / 128-192 / 192-256 and the
+// whole call; the fixed widths before: 0.380 / 1.112 / 2.256, 3.775):
+//   cap 12: 0.344 / 1.042 / 2.180, 3.673;  16: 0.375 / 1.048 / 2.166, 3.700;
+//   20: 0.380 / 1.095 / 2.209, 3.756;  32: 0.377 / 1.146 / 2.322, 3.928;
+// complex128 (fixed widths 0.755 / 2.499 / 5.450, 8.867) at a cap set at run
+// time (about 1% slower than a compile-time cap for float64): 8: 0.744 /
+// 2.459 / 5.384, 8.764;  10: 0.764 / 2.466 / 5.301, 8.705;  12: 0.786 / 2.514
+// / 5.363, 8.828;  16: 0.811 / 2.600 / 5.503, 9.080.
+template <typename T, int NB>
+__host__ __device__ constexpr int bat_cap() {
+    constexpr int c = is_cplx<T>::value ? 10 : 12;
+    return NB > c ? NB : c;
+}
+
 // Trailing tiles in flight per wave (the ring of the kernel's trailing
 // loop).  Measured on the 16384-matrix batch (ms, f64 / c128): 2 tiles 3.76-
 // 3.82 / 8.89, 3 tiles 3.78-3.83 / 8.91-8.92, 4 tiles 3.90-3.96 / 9.65-9.67;
@@ -134,9 +154,11 @@ __device__ __forceinline__ void bat_wave_best(double best, int key, double* r_be
 template <typename T, int NP, int NB>
 __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
     constexpr int NW = NP / 64;
-    extern __shared__ double bat_lds[];     // U_j, V_j of local slot s at [2 (j NP + s)] and [2 (j NP + s) + 1]
+    constexpr int CAPP = NB * NP;            // pairs of panel values the LDS holds
+    constexpr int BCAP = bat_cap<T, NB>();   // pairs per panel at most
+    extern __shared__ double bat_lds[];     // U_j, V_j of local slot s at [2 (j w + s)] and [2 (j w + s) + 1]
     T* Pp = reinterpret_cast<T*>(bat_lds);
-    __shared__ int Qpan[2][2 * NB + 2];     // slot of logical k0 + q, by pair-step parity
+    __shared__ int Qpan[2][2 * BCAP + 2];   // slot of logical k0 + q, by pair-step parity
     __shared__ int Qs[NP];                  // trailing update: local slot of logical lo + r
     __shared__ double r_best[2][NW];        // per-wave argmax, by pair-step parity
     __shared__ int r_key[2][NW];
@@ -182,23 +204,26 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
         Phase<T> ph;
         bool dead = false;
 
-        for (int k0 = 0; k0 < n; k0 += 2 * NB) {
-            const int np = min(NB, (n - k0) / 2);
-            const int s = t;                 // local slot
+        for (int k0 = 0, np = 0; k0 < n; k0 += 2 * np) {
+            const int w = n - k0;            // live width: the slot stride of the panel vectors
+            np = min(min(BCAP, CAPP / w), w / 2);      // >= 1: w >= 2, CAPP > w
             const int x = k0 + t;            // slot: index in src
             const bool own = x < n;
+            const int s = own ? t : 0;       // local slot (a valid one for masked reads)
             const int xs = own ? x : k0;     // a valid slot for masked reads
             int lg = x;                      // logical index of the own slot
             __syncthreads();                 // the previous panel's LDS reads are done
-            for (int q = t; q < 2 * NB + 2; q += NP) Qpan[0][q] = k0 + q;
+            for (int q = t; q < 2 * np + 2; q += NP) Qpan[0][q] = k0 + q;
+            if (own) {
 #pragma unroll 4
-            for (int j = 0; j < NB; ++j) { Pp[2 * (j * NP + s)] = zero<T>(); Pp[2 * (j * NP + s) + 1] = zero<T>(); }
+                for (int j = 0; j < np; ++j) { Pp[2 * (j * w + s)] = zero<T>(); Pp[2 * (j * w + s) + 1] = zero<T>(); }
+            }
             // first pivot row: row k0 of src (nothing pending)
             {
                 double best = -1.0;
                 const T r0v = bat_w(src, ld, k0, xs);
                 const T r = bat_sel(own && x > k0, r0v);
-                Pp[2 * s + 1] = r;                                   // V_0
+                if (own) Pp[2 * s + 1] = r;                          // V_0
                 const double m = bat_mag(r);
                 if (own && x > k0 && m == m) best = m;
                 bat_wave_best(best, lg * 512 + s, r_best[0], r_key[0]);
@@ -218,14 +243,14 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
                 int p, sp;
                 if (gb >= 0.0) { p = gk >> 9; sp = k0 + (gk & 511); }
                 else { p = k + 1; sp = sa; }                         // no finite candidate (NaN row)
-                const T piv = Pp[2 * (i * NP + (sp - k0)) + 1];      // row k at column p
+                const T piv = Pp[2 * (i * w + (sp - k0)) + 1];       // row k at column p
                 if (is_zero(piv)) { dead = true; break; }            // the same decision in every thread
                 const bool sw = p != k + 1;
                 la.mul(absd(piv));
                 ph.mul(piv);
                 if (sw) ph.flip();
                 // the next step's map: interchange k+1 <-> p
-                for (int q = t; q < 2 * NB + 2; q += NP) {
+                for (int q = t; q < 2 * np + 2; q += NP) {
                     int v = Qpan[cb][q];
                     if (q == k + 1 - k0) v = sp;
                     else if (sw && q == p - k0) v = sa;
@@ -242,20 +267,20 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
                 const int s1 = y1 - k0, s2 = y2 - k0;
                 T r1 = zero<T>(), rl = zero<T>(), rk = zero<T>();
                 for (int j = 0; j < i; ++j) {
-                    const T uj = Pp[2 * (j * NP + s)], vj = Pp[2 * (j * NP + s) + 1];
-                    const T b1u = Pp[2 * (j * NP + s1)], b1v = Pp[2 * (j * NP + s1) + 1];
-                    const T b2u = Pp[2 * (j * NP + s2)], b2v = Pp[2 * (j * NP + s2) + 1];
+                    const T uj = Pp[2 * (j * w + s)], vj = Pp[2 * (j * w + s) + 1];
+                    const T b1u = Pp[2 * (j * w + s1)], b1v = Pp[2 * (j * w + s1) + 1];
+                    const T b2u = Pp[2 * (j * w + s2)], b2v = Pp[2 * (j * w + s2) + 1];
                     r1 = r1 + b1u * vj - b1v * uj;
                     rl = rl + b2u * vj - b2v * uj;
                     rk = rk + b1u * b2v - b1v * b2u;
                 }
                 const T inv = recip(piv);
                 const T u = bat_sel(own && lg >= k + 2, (w1 + r1) * inv);
-                const T vi = Pp[2 * (i * NP + s) + 1];               // V_i at the own slot
-                Pp[2 * (i * NP + s)] = u;                            // U_i
+                const T vi = Pp[2 * (i * w + s) + 1];                // V_i at the own slot
+                if (own) Pp[2 * (i * w + s)] = u;                    // U_i
                 if (!has_next) break;
                 const T u_kl = (w12 + rk) * inv;                     // U_i at row k+2
-                const T v_kl = Pp[2 * (i * NP + s2) + 1];            // V_i at row k+2
+                const T v_kl = Pp[2 * (i * w + s2) + 1];             // V_i at row k+2
                 double best = -1.0;
                 T r = zero<T>();
                 if (own && lg >= k + 3) {
@@ -263,7 +288,7 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
                     const double m = bat_mag(r);
                     if (m == m) best = m;
                 }
-                Pp[2 * ((i + 1) * NP + s) + 1] = r;                  // V_{i+1} = row k+2
+                if (own) Pp[2 * ((i + 1) * w + s) + 1] = r;          // V_{i+1} = row k+2
                 bat_wave_best(best, lg * 512 + s, r_best[cb ^ 1], r_key[cb ^ 1]);
                 __syncthreads();
             }
@@ -292,13 +317,18 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
 #pragma unroll
                 for (int e = 0; e < 4; ++e) w4[e] = src[ix[e]];
             };
-            // full panels (np == NB) need no masks on the K index: the
-            // tile loop is compiled once with them and once without.  Each
+            // K runs over the panel's pairs in groups of two (one MFMA K
+            // range of four: U_j, V_j, U_{j+1}, V_{j+1}), GU groups at a
+            // time (four real, two complex: the complex form's registers
+            // would otherwise cost a wave per SIMD), every operand read of
+            // the GU issued ahead of their products; an odd pair count ends
+            // with a masked group.  Each
             // wave keeps a ring of BAT_RING tiles in flight: a slot's next
             // tile is gathered right after the slot's products, so the
             // other slots' products cover the gather.
-            auto trail = [&](auto full_c) {
-            constexpr bool FULL = decltype(full_c)::value;
+            const int gfull = np >> 1;
+            constexpr int GU = is_cplx<T>::value ? 2 : 4;
+            {
             int TI[BAT_RING], TJ[BAT_RING];
             T W4[BAT_RING][4];
             int SG[BAT_RING][4];
@@ -322,19 +352,15 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
                     const int sA = Qs[min(16 * I + lr, m - 1)], sB = Qs[min(cc, m - 1)];
                     bat_d4 acc = {0.0, 0.0, 0.0, 0.0};
                     bat_d4 aci = {0.0, 0.0, 0.0, 0.0};     // complex: the imaginary part
-                    // unrolled to the panel width so that every operand read is
-                    // issued ahead of the products (pairs past np are masked);
-                    // complex products as four real ones:
+                    // lane (lr, lq) takes K index lq of each group: pair
+                    // 2 g + lq / 2, its U (lq even) or V; complex products as
+                    // four real ones:
                     //   Re += Re a Re b - Im a Im b,  Im += Re a Im b + Im a Re b
-#pragma unroll
-                    for (int g = 0; g < (2 * NB + 3) / 4; ++g) {
-                        const int q = 4 * g + lq;                        // K index: pair q / 2
-                        const int j = q >> 1;
-                        const bool on = FULL || j < np;
-                        const int jj = on ? j : 0;
-                        const T a = bat_sel(on, Pp[2 * (jj * NP + sA) + (q & 1)]);         // U_j, V_j at the row
-                        const T b0 = Pp[2 * (jj * NP + sB) + ((q & 1) ^ 1)];              // V_j, U_j at the column
-                        const T b = bat_sel(on, (q & 1) ? -b0 : b0);
+                    const int par = lq & 1;
+                    const T* pa = Pp + 2 * ((lq >> 1) * w + sA) + par;          // U_j, V_j at the row
+                    const T* pb = Pp + 2 * ((lq >> 1) * w + sB) + (par ^ 1);    // V_j, U_j at the column
+                    auto group = [&](T a, T b0) {
+                        const T b = par ? -b0 : b0;
                         if constexpr (is_cplx<T>::value) {
                             acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.re, acc, 0, 0, 0);
                             acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, b.im, acc, 0, 0, 0);
@@ -343,15 +369,30 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
                         } else {
                             acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
                         }
+                    };
+                    const int gs = 4 * w;                 // elements of T per group of two pairs
+                    int g = 0;
+                    for (; g + GU <= gfull; g += GU) {
+                        T a4[GU], b4[GU];
+#pragma unroll
+                        for (int u = 0; u < GU; ++u) { a4[u] = pa[(g + u) * gs]; b4[u] = pb[(g + u) * gs]; }
+#pragma unroll
+                        for (int u = 0; u < GU; ++u) group(a4[u], b4[u]);
+                    }
+                    for (; g < gfull; ++g) group(pa[g * gs], pb[g * gs]);
+                    if (np & 1) {                         // the last pair alone: lanes lq >= 2 masked
+                        const bool on = lq < 2;
+                        const int o = g * gs - (on ? 0 : 2 * w);   // masked lanes read pair np - 1
+                        group(bat_sel(on, pa[o]), bat_sel(on, pb[o]));
                     }
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
                         const int rr = 16 * I + lq + 4 * e;
                         if (rr < m && cc < m && cc > rr) {
-                            const T w = bat_apply(SG[Q][e], W4[Q][e]);
+                            const T stale = bat_apply(SG[Q][e], W4[Q][e]);
                             T v;
-                            if constexpr (is_cplx<T>::value) v = T{w.re + acc[e], w.im + aci[e]};
-                            else v = w + acc[e];
+                            if constexpr (is_cplx<T>::value) v = T{stale.re + acc[e], stale.im + aci[e]};
+                            else v = stale + acc[e];
                             dst[(unsigned)((lo + rr) * NP + (lo + cc))] = v;
                         }
                     }
@@ -362,9 +403,7 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
                     }
                 });
             }
-            };
-            if (np == NB) trail(std::true_type{});
-            else trail(std::false_type{});
+            }
             src = dst;
             ld = NP;
             dst = (dst == slab0) ? slab1 : slab0;
