@@ -113,6 +113,27 @@ __device__ __forceinline__ T bat_apply(int sg, T v) { return sg > 0 ? v : (sg < 
 // time (about 1% slower than a compile-time cap for float64): 8: 0.744 /
 // 2.459 / 5.384, 8.764;  10: 0.764 / 2.466 / 5.301, 8.705;  12: 0.786 / 2.514
 // / 5.363, 8.828;  16: 0.811 / 2.600 / 5.503, 9.080.
+// Pairs of panel values the LDS holds: NB pairs over NP slots, except at
+// the built-in widths of the largest buckets, where the capacity is the most
+// that still lets one more workgroup onto a CU (160 KB of LDS per CU, the
+// kernel's static LDS beside it): float64 NP = 256 four workgroups (2432
+// pairs, 38.9 KB, against 3072), NP = 192 five (1920 against 2304),
+// complex128 NP = 256 three (1600 against 2048).  The panels of the wider
+// matrices start a pair or two narrower; the extra matrix per CU hides more
+// of the pair steps' latency.  Measured on the 16384-matrix batch (ms per
+// bucket, full capacity / this): f64 128-192 1.043 / 0.989, 192-256 2.183
+// / 2.107 (4 workgroups still fit at 2464 pairs: 2.102; 5 do not at 1968 for
+// NP = 192: 1.150); c128 192-256 5.305 / 5.127-5.186 (3 do not fit at 1648:
+// 5.816).  For c128 NP = 192 four workgroups (1216 pairs) measured no faster
+// (2.515 against 2.457-2.498 ms).
+template <typename T, int NP, int NB>
+__host__ __device__ constexpr int bat_capp() {
+    if (!is_cplx<T>::value && NP == 256 && NB == 12) return 2432;
+    if (!is_cplx<T>::value && NP == 192 && NB == 12) return 1920;
+    if (is_cplx<T>::value && NP == 256 && NB == 8) return 1600;
+    return NB * NP;
+}
+
 template <typename T, int NB>
 __host__ __device__ constexpr int bat_cap() {
     constexpr int c = is_cplx<T>::value ? 10 : 12;
@@ -154,7 +175,7 @@ __device__ __forceinline__ void bat_wave_best(double best, int key, double* r_be
 template <typename T, int NP, int NB>
 __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
     constexpr int NW = NP / 64;
-    constexpr int CAPP = NB * NP;            // pairs of panel values the LDS holds
+    constexpr int CAPP = bat_capp<T, NP, NB>();   // pairs of panel values the LDS holds
     constexpr int BCAP = bat_cap<T, NB>();   // pairs per panel at most
     extern __shared__ double bat_lds[];     // U_j, V_j of local slot s at [2 (j w + s)] and [2 (j w + s) + 1]
     T* Pp = reinterpret_cast<T*>(bat_lds);
@@ -423,7 +444,7 @@ namespace {
 int bat_num_cu() { return device_cu_count(); }
 
 template <typename T, int NP, int NB>
-constexpr size_t bat_lds_bytes() { return sizeof(T) * 2 * NB * NP; }
+constexpr size_t bat_lds_bytes() { return sizeof(T) * 2 * bat_capp<T, NP, NB>(); }
 
 // resident grid (workgroups per CU x CUs); also sets the kernel's LDS limit,
 // which the occupancy query needs
