@@ -99,20 +99,6 @@ __device__ __forceinline__ unsigned bat_ix(int ld, int r, int c, int& sg) {
 template <typename T>
 __device__ __forceinline__ T bat_apply(int sg, T v) { return sg > 0 ? v : (sg < 0 ? -v : zero<T>()); }
 
-// Pairs per panel.  The LDS holds NB NP pairs of panel values (the
-// instantiation's width NB over NP slots); a panel of live width w = n - k0
-// stores its vectors with slot stride w and takes min(cap, NB NP / w) pairs,
-// so the panels widen as the matrix shrinks (and a matrix narrower than NP
-// starts wider): fewer trailing passes at the same occupancy, against longer
-// chains of corrections in the pair steps, hence the cap.  Measured on the
-// 16384-matrix batch (ms, f64 buckets 64-128 / 128-192 / 192-256 and the
-// whole call; the fixed widths before: 0.380 / 1.112 / 2.256, 3.775):
-//   cap 12: 0.344 / 1.042 / 2.180, 3.673;  16: 0.375 / 1.048 / 2.166, 3.700;
-//   20: 0.380 / 1.095 / 2.209, 3.756;  32: 0.377 / 1.146 / 2.322, 3.928;
-// complex128 (fixed widths 0.755 / 2.499 / 5.450, 8.867) at a cap set at run
-// time (about 1% slower than a compile-time cap for float64): 8: 0.744 /
-// 2.459 / 5.384, 8.764;  10: 0.764 / 2.466 / 5.301, 8.705;  12: 0.786 / 2.514
-// / 5.363, 8.828;  16: 0.811 / 2.600 / 5.503, 9.080.
 // Pairs of panel values the LDS holds: NB pairs over NP slots, except at
 // the built-in widths of the largest buckets, where the capacity is the most
 // that still lets one more workgroup onto a CU (160 KB of LDS per CU, the
@@ -134,6 +120,20 @@ __host__ __device__ constexpr int bat_capp() {
     return NB * NP;
 }
 
+// Pairs per panel.  The LDS holds NB NP pairs of panel values (the
+// instantiation's width NB over NP slots); a panel of live width w = n - k0
+// stores its vectors with slot stride w and takes min(cap, NB NP / w) pairs,
+// so the panels widen as the matrix shrinks (and a matrix narrower than NP
+// starts wider): fewer trailing passes at the same occupancy, against longer
+// chains of corrections in the pair steps, hence the cap.  Measured on the
+// 16384-matrix batch (ms, f64 buckets 64-128 / 128-192 / 192-256 and the
+// whole call; the fixed widths before: 0.380 / 1.112 / 2.256, 3.775):
+//   cap 12: 0.344 / 1.042 / 2.180, 3.673;  16: 0.375 / 1.048 / 2.166, 3.700;
+//   20: 0.380 / 1.095 / 2.209, 3.756;  32: 0.377 / 1.146 / 2.322, 3.928;
+// complex128 (fixed widths 0.755 / 2.499 / 5.450, 8.867) at a cap set at run
+// time (about 1% slower than a compile-time cap for float64): 8: 0.744 /
+// 2.459 / 5.384, 8.764;  10: 0.764 / 2.466 / 5.301, 8.705;  12: 0.786 / 2.514
+// / 5.363, 8.828;  16: 0.811 / 2.600 / 5.503, 9.080.
 template <typename T, int NB>
 __host__ __device__ constexpr int bat_cap() {
     constexpr int c = is_cplx<T>::value ? 10 : 12;
@@ -172,8 +172,15 @@ __device__ __forceinline__ void bat_wave_best(double best, int key, double* r_be
 
 }  // namespace
 
+// Waves per SIMD the register allocation must allow: complex128 needs three
+// for its occupancy-sized LDS capacity at NP = 256 (three workgroups of four
+// waves per CU); float64 fits five unconstrained.
+template <typename T>
+constexpr int bat_waves() { return is_cplx<T>::value ? 3 : 1; }
+
 template <typename T, int NP, int NB>
-__global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
+__global__ void __launch_bounds__(NP) __attribute__((amdgpu_waves_per_eu(bat_waves<T>(), 8)))
+pf_bat_kernel(BatchDesc d) {
     constexpr int NW = NP / 64;
     constexpr int CAPP = bat_capp<T, NP, NB>();   // pairs of panel values the LDS holds
     constexpr int BCAP = bat_cap<T, NB>();   // pairs per panel at most
@@ -264,12 +271,19 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
                 int p, sp;
                 if (gb >= 0.0) { p = gk >> 9; sp = k0 + (gk & 511); }
                 else { p = k + 1; sp = sa; }                         // no finite candidate (NaN row)
-                const T piv = Pp[2 * (i * w + (sp - k0)) + 1];       // row k at column p
-                if (is_zero(piv)) { dead = true; break; }            // the same decision in every thread
                 const bool sw = p != k + 1;
-                la.mul(absd(piv));
-                ph.mul(piv);
-                if (sw) ph.flip();
+                const int y1 = sp, y2 = (p == k + 2) ? sa : sb;
+                // the stale rows k+1 and k+2 at the own slot go out first (the
+                // addresses are valid whatever the pivot's value, and on the
+                // last pair of the matrix too); the pivot's value and the
+                // zero test follow, in the shadow of these loads
+                const T w1 = bat_w(src, ld, y1, xs);
+                const T w2 = bat_w(src, ld, y2, xs);
+                const T w12 = bat_w(src, ld, y1, y2);
+                const int s1 = y1 - k0, s2 = y2 - k0;
+                const T piv = Pp[2 * (i * w + s1) + 1];              // row k at column p
+                const T vi = Pp[2 * (i * w + s) + 1];                // V_i at the own slot
+                const T v_kl = Pp[2 * (i * w + s2) + 1];             // V_i at row k+2
                 // the next step's map: interchange k+1 <-> p
                 for (int q = t; q < 2 * np + 2; q += NP) {
                     int v = Qpan[cb][q];
@@ -279,13 +293,7 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
                 }
                 if (x == sp) lg = k + 1;
                 else if (sw && x == sa) lg = p;
-                const int y1 = sp, y2 = (p == k + 2) ? sa : sb;
-                if (k + 2 >= n) break;                               // last pair of the matrix
-
-                const T w1 = bat_w(src, ld, y1, xs);
-                const T w2 = bat_w(src, ld, y2, xs);
-                const T w12 = bat_w(src, ld, y1, y2);
-                const int s1 = y1 - k0, s2 = y2 - k0;
+                const T inv = recip(piv);                            // inf for a zero pivot, never used then
                 T r1 = zero<T>(), rl = zero<T>(), rk = zero<T>();
                 for (int j = 0; j < i; ++j) {
                     const T uj = Pp[2 * (j * w + s)], vj = Pp[2 * (j * w + s) + 1];
@@ -295,13 +303,15 @@ __global__ void __launch_bounds__(NP) pf_bat_kernel(BatchDesc d) {
                     rl = rl + b2u * vj - b2v * uj;
                     rk = rk + b1u * b2v - b1v * b2u;
                 }
-                const T inv = recip(piv);
                 const T u = bat_sel(own && lg >= k + 2, (w1 + r1) * inv);
-                const T vi = Pp[2 * (i * w + s) + 1];                // V_i at the own slot
+                if (is_zero(piv)) { dead = true; break; }            // the same decision in every thread
+                la.mul(absd(piv));
+                ph.mul(piv);
+                if (sw) ph.flip();
+                if (k + 2 >= n) break;                               // last pair of the matrix
                 if (own) Pp[2 * (i * w + s)] = u;                    // U_i
                 if (!has_next) break;
                 const T u_kl = (w12 + rk) * inv;                     // U_i at row k+2
-                const T v_kl = Pp[2 * (i * w + s2) + 1];             // V_i at row k+2
                 double best = -1.0;
                 T r = zero<T>();
                 if (own && lg >= k + 3) {
