@@ -424,7 +424,7 @@ pf_bat_kernel(BatchDesc d) {
                             T v;
                             if constexpr (is_cplx<T>::value) v = T{stale.re + acc[e], stale.im + aci[e]};
                             else v = stale + acc[e];
-                            dst[(unsigned)((lo + rr) * NP + (lo + cc))] = v;
+                            dst[(unsigned)(rr * NP + cc)] = v;
                         }
                     }
                     const int tn = tt + BAT_RING * NW;             // this slot's next tile
@@ -435,7 +435,10 @@ pf_bat_kernel(BatchDesc d) {
                 });
             }
             }
-            src = dst;
+            // the trailing matrix sits at the slab's origin, so that a tile's
+            // 16-entry row segments are whole 128-byte lines; src is offset
+            // to read it by matrix indices (all >= lo from here on)
+            src = dst - (lo * NP + lo);
             ld = NP;
             dst = (dst == slab0) ? slab1 : slab0;
         }
