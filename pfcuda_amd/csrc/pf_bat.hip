@@ -140,24 +140,6 @@ __host__ __device__ constexpr int bat_cap() {
     return NB > c ? NB : c;
 }
 
-// Trailing tiles in flight per wave (the ring of the kernel's trailing
-// loop).  Measured on the 16384-matrix batch (ms, f64 / c128): 2 tiles 3.76-
-// 3.82 / 8.89, 3 tiles 3.78-3.83 / 8.91-8.92, 4 tiles 3.90-3.96 / 9.65-9.67;
-// the earlier loop, which gathered the next tile before the current one's
-// products, 3.88 / 9.09.
-#ifndef BAT_RING
-#define BAT_RING 2
-#endif
-
-// f(std::integral_constant<int, I>) for I = I0 .. N-1
-template <int I, int N, typename F>
-__device__ __forceinline__ void bat_sfor(F&& f) {
-    if constexpr (I < N) {
-        f(std::integral_constant<int, I>{});
-        bat_sfor<I + 1, N>(f);
-    }
-}
-
 // pivot-search magnitude: |x| for real, |z|^2 for complex (NaN stays NaN)
 __device__ __forceinline__ double bat_mag(double x) { return fabs(x); }
 __device__ __forceinline__ double bat_mag(c128 z) { return z.re * z.re + z.im * z.im; }
@@ -353,87 +335,93 @@ pf_bat_kernel(BatchDesc d) {
             // time (four real, two complex: the complex form's registers
             // would otherwise cost a wave per SIMD), every operand read of
             // the GU issued ahead of their products; an odd pair count ends
-            // with a masked group.  Each
-            // wave keeps a ring of BAT_RING tiles in flight: a slot's next
-            // tile is gathered right after the slot's products, so the
-            // other slots' products cover the gather.
+            // with a masked group.
             const int gfull = np >> 1;
             constexpr int GU = is_cplx<T>::value ? 2 : 4;
-            {
-            int TI[BAT_RING], TJ[BAT_RING];
-            T W4[BAT_RING][4];
-            int SG[BAT_RING][4];
-            bat_sfor<0, BAT_RING>([&](auto qc) {
-                constexpr int Q = decltype(qc)::value;
-                TI[Q] = 0; TJ[Q] = 0;
+            // one tile: the products, then the stale entries added and the
+            // results stored
+            auto tile = [&](int I, int J, const T (&w4)[4], const int (&sg4)[4]) {
+                const int cc = 16 * J + lr;
+                const int sA = Qs[min(16 * I + lr, m - 1)], sB = Qs[min(cc, m - 1)];
+                bat_d4 acc = {0.0, 0.0, 0.0, 0.0};
+                bat_d4 aci = {0.0, 0.0, 0.0, 0.0};     // complex: the imaginary part
+                // lane (lr, lq) takes K index lq of each group: pair
+                // 2 g + lq / 2, its U (lq even) or V; complex products as
+                // four real ones:
+                //   Re += Re a Re b - Im a Im b,  Im += Re a Im b + Im a Re b
+                const int par = lq & 1;
+                const T* pa = Pp + 2 * ((lq >> 1) * w + sA) + par;          // U_j, V_j at the row
+                const T* pb = Pp + 2 * ((lq >> 1) * w + sB) + (par ^ 1);    // V_j, U_j at the column
+                auto group = [&](T a, T b0) {
+                    const T b = par ? -b0 : b0;
+                    if constexpr (is_cplx<T>::value) {
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.re, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, b.im, acc, 0, 0, 0);
+                        aci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.im, aci, 0, 0, 0);
+                        aci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b.re, aci, 0, 0, 0);
+                    } else {
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+                    }
+                };
+                const int gs = 4 * w;                 // elements of T per group of two pairs
+                int g = 0;
+                for (; g + GU <= gfull; g += GU) {
+                    T a4[GU], b4[GU];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) { W4[Q][e] = zero<T>(); SG[Q][e] = 0; }
-                if (wv + Q * NW < ntile) {
-                    tile_ij(wv + Q * NW, TI[Q], TJ[Q]);
-                    gather(TI[Q], TJ[Q], W4[Q], SG[Q]);
+                    for (int u = 0; u < GU; ++u) { a4[u] = pa[(g + u) * gs]; b4[u] = pb[(g + u) * gs]; }
+#pragma unroll
+                    for (int u = 0; u < GU; ++u) group(a4[u], b4[u]);
                 }
-            });
-            for (int base = wv; base < ntile; base += BAT_RING * NW) {
-                bat_sfor<0, BAT_RING>([&](auto qc) {
-                    constexpr int Q = decltype(qc)::value;
-                    const int tt = base + Q * NW;
-                    if (tt >= ntile) return;
-                    const int I = TI[Q], J = TJ[Q];
-                    const int cc = 16 * J + lr;
-                    const int sA = Qs[min(16 * I + lr, m - 1)], sB = Qs[min(cc, m - 1)];
-                    bat_d4 acc = {0.0, 0.0, 0.0, 0.0};
-                    bat_d4 aci = {0.0, 0.0, 0.0, 0.0};     // complex: the imaginary part
-                    // lane (lr, lq) takes K index lq of each group: pair
-                    // 2 g + lq / 2, its U (lq even) or V; complex products as
-                    // four real ones:
-                    //   Re += Re a Re b - Im a Im b,  Im += Re a Im b + Im a Re b
-                    const int par = lq & 1;
-                    const T* pa = Pp + 2 * ((lq >> 1) * w + sA) + par;          // U_j, V_j at the row
-                    const T* pb = Pp + 2 * ((lq >> 1) * w + sB) + (par ^ 1);    // V_j, U_j at the column
-                    auto group = [&](T a, T b0) {
-                        const T b = par ? -b0 : b0;
-                        if constexpr (is_cplx<T>::value) {
-                            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.re, acc, 0, 0, 0);
-                            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, b.im, acc, 0, 0, 0);
-                            aci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.im, aci, 0, 0, 0);
-                            aci = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b.re, aci, 0, 0, 0);
-                        } else {
-                            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-                        }
-                    };
-                    const int gs = 4 * w;                 // elements of T per group of two pairs
-                    int g = 0;
-                    for (; g + GU <= gfull; g += GU) {
-                        T a4[GU], b4[GU];
+                for (; g < gfull; ++g) group(pa[g * gs], pb[g * gs]);
+                if (np & 1) {                         // the last pair alone: lanes lq >= 2 masked
+                    const bool on = lq < 2;
+                    const int o = g * gs - (on ? 0 : 2 * w);   // masked lanes read pair np - 1
+                    group(bat_sel(on, pa[o]), bat_sel(on, pb[o]));
+                }
+                // unconditional stores: the entries outside the new upper
+                // triangle (rr >= m, cc >= m or cc <= rr) land inside the
+                // slab (rr, cc < 16 mt <= NP) where nothing reads them.  A
+                // store under a branch made the compiler wait for every
+                // memory operation in flight (this wave's next gathers
+                // included) before reusing its data registers; the four
+                // values are formed first so that they sit in four
+                // register pairs and the stores issue back to back.
+                T ov[4];
 #pragma unroll
-                        for (int u = 0; u < GU; ++u) { a4[u] = pa[(g + u) * gs]; b4[u] = pb[(g + u) * gs]; }
+                for (int e = 0; e < 4; ++e) {
+                    const T stale = bat_apply(sg4[e], w4[e]);
+                    if constexpr (is_cplx<T>::value) ov[e] = T{stale.re + acc[e], stale.im + aci[e]};
+                    else ov[e] = stale + acc[e];
+                }
 #pragma unroll
-                        for (int u = 0; u < GU; ++u) group(a4[u], b4[u]);
-                    }
-                    for (; g < gfull; ++g) group(pa[g * gs], pb[g * gs]);
-                    if (np & 1) {                         // the last pair alone: lanes lq >= 2 masked
-                        const bool on = lq < 2;
-                        const int o = g * gs - (on ? 0 : 2 * w);   // masked lanes read pair np - 1
-                        group(bat_sel(on, pa[o]), bat_sel(on, pb[o]));
-                    }
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const int rr = 16 * I + lq + 4 * e;
-                        if (rr < m && cc < m && cc > rr) {
-                            const T stale = bat_apply(SG[Q][e], W4[Q][e]);
-                            T v;
-                            if constexpr (is_cplx<T>::value) v = T{stale.re + acc[e], stale.im + aci[e]};
-                            else v = stale + acc[e];
-                            dst[(unsigned)(rr * NP + cc)] = v;
-                        }
-                    }
-                    const int tn = tt + BAT_RING * NW;             // this slot's next tile
-                    if (tn < ntile) {
-                        tile_ij(tn, TI[Q], TJ[Q]);
-                        gather(TI[Q], TJ[Q], W4[Q], SG[Q]);
-                    }
-                });
-            }
+                for (int e = 0; e < 4; ++e) dst[(unsigned)((16 * I + lq + 4 * e) * NP + cc)] = ov[e];
+            };
+            // Wave wv takes tiles wv, wv + NW, ... (q = 0 .. ntw - 1) in two
+            // slots, each tile's stale entries gathered one tile ahead: the
+            // steady-state loop handles two tiles with no branch around any
+            // memory access (the gathers past the last tile read the last
+            // tile again), so the compiler's waits are for the tile at hand
+            // only, the other slot's gathers stay in flight.
+            const int ntw = wv < ntile ? (ntile - wv + NW - 1) / NW : 0;
+            if (ntw > 0) {
+                auto tix = [&](int q) { return min(wv + q * NW, ntile - 1); };
+                int I0, J0, I1, J1;
+                T W0[4], W1[4];
+                int S0[4], S1[4];
+                tile_ij(tix(0), I0, J0);
+                gather(I0, J0, W0, S0);
+                tile_ij(tix(1), I1, J1);
+                gather(I1, J1, W1, S1);
+                int q = 0;
+                for (; q + 2 <= ntw; q += 2) {
+                    tile(I0, J0, W0, S0);
+                    tile_ij(tix(q + 2), I0, J0);
+                    gather(I0, J0, W0, S0);
+                    tile(I1, J1, W1, S1);
+                    tile_ij(tix(q + 3), I1, J1);
+                    gather(I1, J1, W1, S1);
+                }
+                if (q < ntw) tile(I0, J0, W0, S0);
             }
             // the trailing matrix sits at the slab's origin, so that a tile's
             // 16-entry row segments are whole 128-byte lines; src is offset
