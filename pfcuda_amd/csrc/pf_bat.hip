@@ -74,14 +74,20 @@ namespace {
 
 typedef double bat_d4 __attribute__((ext_vector_type(4)));
 
+// Element e of a matrix or slab whose base is wave-uniform, by a 32-bit byte
+// offset (every offset within one matrix or slab stays far below 2^32), so
+// that the access takes the scalar-base form with one 32-bit vector offset.
+template <typename T>
+__device__ __forceinline__ const T& bat_at(const T* W, unsigned e) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(W) + e * (unsigned)sizeof(T));
+}
+
 // Entry (r, c) of a matrix whose upper triangle is stored with row stride
 // ld; the load is unconditional (the diagonal when r == c).
-// (Offsets within one matrix or slab stay below 256^2 elements: 32-bit
-// arithmetic from the wave-uniform base.)
 template <typename T>
 __device__ __forceinline__ T bat_w(const T* W, int ld, int r, int c) {
     const int a = r < c ? r : c, b = r < c ? c : r;
-    const T v = W[(unsigned)(a * ld + b)];
+    const T v = bat_at(W, (unsigned)(a * ld + b));
     return r < c ? v : (r > c ? -v : zero<T>());
 }
 
@@ -89,15 +95,21 @@ template <typename T>
 __device__ __forceinline__ T bat_sel(bool on, T v) { return on ? v : zero<T>(); }
 
 // The same entry split in two, so that a run of loads can be issued before
-// any of them is used: the element index, and a sign code from the indices
-// alone (+1 above the diagonal, -1 below, 0 on it).
-__device__ __forceinline__ unsigned bat_ix(int ld, int r, int c, int& sg) {
-    sg = r < c ? 1 : (r > c ? -1 : 0);
+// any of them is used: the element index, and whether the entry lies below
+// the diagonal (its value is then the negated stored one).  On the diagonal
+// (r == c) the stored value comes back as it is: the trailing update only
+// reads an entry there for a result nothing reads.
+__device__ __forceinline__ unsigned bat_ix(int ld, int r, int c, bool& below) {
+    below = r > c;
     const int a = r < c ? r : c, b = r < c ? c : r;
     return (unsigned)(a * ld + b);
 }
-template <typename T>
-__device__ __forceinline__ T bat_apply(int sg, T v) { return sg > 0 ? v : (sg < 0 ? -v : zero<T>()); }
+// v negated when `below`, as a flip of the sign bit(s)
+__device__ __forceinline__ double bat_flip(bool below, double v) {
+    return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, v) ^
+                                          ((unsigned long long)below << 63));
+}
+__device__ __forceinline__ c128 bat_flip(bool below, c128 v) { return c128{bat_flip(below, v.re), bat_flip(below, v.im)}; }
 
 // Pairs of panel values the LDS holds: NB pairs over NP slots, except at
 // the built-in widths of the largest buckets, where the capacity is the most
@@ -317,10 +329,14 @@ pf_bat_kernel(BatchDesc d) {
             if (lo >= n) break;
 
             // ---- trailing update of [lo, n) into dst, 16 x 16 upper tiles ----
-            if (own && lg >= lo) Qs[lg - lo] = s;
-            __syncthreads();
             const int m = n - lo;
             const int mt = (m + 15) >> 4;
+            if (own && lg >= lo) Qs[lg - lo] = s;
+            // positions m .. 16 mt - 1 (rows and columns past the matrix in
+            // its last tile row and column) point at slot 0: valid reads
+            // for results nothing reads, so that no index needs a clamp
+            if (t >= m && t < 16 * mt) Qs[t] = 0;
+            __syncthreads();
             const int ntile = mt * (mt + 1) / 2;
             auto tile_ij = [&](int tt, int& I, int& J) {
                 int rem = tt;
@@ -329,27 +345,31 @@ pf_bat_kernel(BatchDesc d) {
                 J = I + rem;
             };
             // raw loads now, signs applied where the values are used
-            auto gather = [&](int I, int J, T (&w4)[4], int (&sg4)[4]) {
-                const int scc = Qs[min(16 * J + lr, m - 1)] + k0;
+            auto gather = [&](int I, int J, T (&w4)[4], bool (&bl4)[4]) {
+                const int scc = Qs[16 * J + lr] + k0;
                 unsigned ix[4];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) ix[e] = bat_ix(ld, Qs[min(16 * I + lq + 4 * e, m - 1)] + k0, scc, sg4[e]);
+                for (int e = 0; e < 4; ++e) ix[e] = bat_ix(ld, Qs[16 * I + lq + 4 * e] + k0, scc, bl4[e]);
 #pragma unroll
-                for (int e = 0; e < 4; ++e) w4[e] = src[ix[e]];
+                for (int e = 0; e < 4; ++e) w4[e] = bat_at(src, ix[e]);
             };
             // K runs over the panel's pairs in groups of two (one MFMA K
             // range of four: U_j, V_j, U_{j+1}, V_{j+1}), GU groups at a
             // time (four real, two complex: the complex form's registers
             // would otherwise cost a wave per SIMD), every operand read of
             // the GU issued ahead of their products; an odd pair count ends
-            // with a masked group.
+            // with a masked group.  (Splitting the product in two, U V^T and
+            // V U^T, one accumulation each, so that a lane reads its pair with
+            // one LDS access per side and negates nothing, measured slower:
+            // groups of four pairs leave a panel of ten pairs a half-masked
+            // third group, f64 192-256 bucket 1.91 -> 2.05 ms.)
             const int gfull = np >> 1;
             constexpr int GU = is_cplx<T>::value ? 2 : 4;
             // one tile: the products, then the stale entries added and the
             // results stored
-            auto tile = [&](int I, int J, const T (&w4)[4], const int (&sg4)[4]) {
+            auto tile = [&](int I, int J, const T (&w4)[4], const bool (&bl4)[4]) {
                 const int cc = 16 * J + lr;
-                const int sA = Qs[min(16 * I + lr, m - 1)], sB = Qs[min(cc, m - 1)];
+                const int sA = Qs[16 * I + lr], sB = Qs[cc];
                 bat_d4 acc = {0.0, 0.0, 0.0, 0.0};
                 bat_d4 aci = {0.0, 0.0, 0.0, 0.0};     // complex: the imaginary part
                 // lane (lr, lq) takes K index lq of each group: pair
@@ -396,12 +416,14 @@ pf_bat_kernel(BatchDesc d) {
                 T ov[4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const T stale = bat_apply(sg4[e], w4[e]);
+                    const T stale = bat_flip(bl4[e], w4[e]);
                     if constexpr (is_cplx<T>::value) ov[e] = T{stale.re + acc[e], stale.im + aci[e]};
                     else ov[e] = stale + acc[e];
                 }
 #pragma unroll
-                for (int e = 0; e < 4; ++e) dst[(unsigned)((16 * I + lq + 4 * e) * NP + cc)] = ov[e];
+                for (int e = 0; e < 4; ++e)
+                    *reinterpret_cast<T*>(reinterpret_cast<char*>(dst) +
+                                          (unsigned)((16 * I + lq + 4 * e) * NP + cc) * (unsigned)sizeof(T)) = ov[e];
             };
             // Wave wv takes tiles wv, wv + NW, ... (q = 0 .. ntw - 1) in two
             // slots, each tile's stale entries gathered one tile ahead: the
@@ -415,7 +437,7 @@ pf_bat_kernel(BatchDesc d) {
                 auto tix = [&](int q) { return min(wv + q * NW, ntile - 1); };
                 int TI[S], TJ[S];
                 T WW[S][4];
-                int SS[S][4];
+                bool SS[S][4];
 #pragma unroll
                 for (int u = 0; u < S; ++u) {
                     tile_ij(tix(u), TI[u], TJ[u]);
