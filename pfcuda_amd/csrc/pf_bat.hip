@@ -405,15 +405,15 @@ pf_bat_kernel(BatchDesc d) {
                     const int o = g * gs - (on ? 0 : 2 * w);   // masked lanes read pair np - 1
                     group(bat_sel(on, pa[o]), bat_sel(on, pb[o]));
                 }
-                // unconditional stores: the entries outside the new upper
-                // triangle (rr >= m, cc >= m or cc <= rr) land inside the
-                // slab (rr, cc < 16 mt <= NP) where nothing reads them.  A
-                // store under a branch made the compiler wait for every
-                // memory operation in flight (this wave's next gathers
-                // included) before reusing its data registers; the four
-                // values are formed first so that they sit in four
-                // register pairs and the stores issue back to back.
+                // unconditional stores (a store under a branch made the
+                // compiler wait for every memory operation in flight, this
+                // wave's next gathers included, before reusing its data
+                // registers); the four values are formed first so that they
+                // sit in four register pairs and the stores issue back to
+                // back.  Rows past the matrix (rr < 16 mt <= NP) and the
+                // trash slot lie inside the slab.
                 T ov[4];
+                unsigned ix[4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const T stale = bat_flip(bl4[e], w4[e]);
@@ -421,9 +421,21 @@ pf_bat_kernel(BatchDesc d) {
                     else ov[e] = stale + acc[e];
                 }
 #pragma unroll
+                for (int e = 0; e < 4; ++e) ix[e] = (unsigned)((16 * I + lq + 4 * e) * NP + cc);
+                // a tile on the diagonal or past the matrix's last row or
+                // column sends its entries outside the new upper triangle to
+                // one slot nothing reads (row NP - 1, column 0: below the
+                // diagonal), so that their lines are not written
+                if (!(I < J && 16 * J + 16 <= m)) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int rr = 16 * I + lq + 4 * e;
+                        if (!(cc > rr && cc < m)) ix[e] = (unsigned)((NP - 1) * NP);
+                    }
+                }
+#pragma unroll
                 for (int e = 0; e < 4; ++e)
-                    *reinterpret_cast<T*>(reinterpret_cast<char*>(dst) +
-                                          (unsigned)((16 * I + lq + 4 * e) * NP + cc) * (unsigned)sizeof(T)) = ov[e];
+                    *reinterpret_cast<T*>(reinterpret_cast<char*>(dst) + ix[e] * (unsigned)sizeof(T)) = ov[e];
             };
             // Wave wv takes tiles wv, wv + NW, ... (q = 0 .. ntw - 1) in two
             // slots, each tile's stale entries gathered one tile ahead: the
