@@ -56,9 +56,36 @@ template <typename R> struct C32Regs<cplx<R>> {
     __device__ __forceinline__ void set(int i, cplx<R> x) { re.set(i, x.re); im.set(i, x.im); }
 };
 
-__device__ __forceinline__ double c32_xor32(double x) { return __shfl_xor(x, 32); }
-__device__ __forceinline__ float c32_xor32(float x) { return __shfl_xor(x, 32); }
-template <typename R> __device__ __forceinline__ cplx<R> c32_xor32(cplx<R> z) { return {c32_xor32(z.re), c32_xor32(z.im)}; }
+// The interchange's exchange between the halves, in one v_permlane32_swap
+// per 32-bit part (a VALU operation; the lane shuffle it replaces went
+// through the LDS crossbar): with a = row k+1 and b = row p, the upper half
+// receives b from the lower half (a_x, row p seen from the odd half) and the
+// lower half receives a from the upper half (b_x, row k+1 seen from the even
+// half).  Each result is used only in the half that receives it.
+__device__ __forceinline__ void c32_swap(unsigned a, unsigned b, unsigned& a_x, unsigned& b_x) {
+    const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+    a_x = r[0];   // upper lanes: b's lower half
+    b_x = r[1];   // lower lanes: a's upper half
+}
+__device__ __forceinline__ void c32_swap(float a, float b, float& a_x, float& b_x) {
+    unsigned x, y;
+    c32_swap(__builtin_bit_cast(unsigned, a), __builtin_bit_cast(unsigned, b), x, y);
+    a_x = __builtin_bit_cast(float, x);
+    b_x = __builtin_bit_cast(float, y);
+}
+__device__ __forceinline__ void c32_swap(double a, double b, double& a_x, double& b_x) {
+    const unsigned long long ua = __builtin_bit_cast(unsigned long long, a), ub = __builtin_bit_cast(unsigned long long, b);
+    unsigned lx, ly, hx, hy;
+    c32_swap((unsigned)ua, (unsigned)ub, lx, ly);
+    c32_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), hx, hy);
+    a_x = __builtin_bit_cast(double, ((unsigned long long)hx << 32) | lx);
+    b_x = __builtin_bit_cast(double, ((unsigned long long)hy << 32) | ly);
+}
+template <typename R>
+__device__ __forceinline__ void c32_swap(cplx<R> a, cplx<R> b, cplx<R>& a_x, cplx<R>& b_x) {
+    c32_swap(a.re, b.re, a_x.re, b_x.re);
+    c32_swap(a.im, b.im, a_x.im, b_x.im);
+}
 
 __device__ __forceinline__ double c32_lane(double x, int l) { return readlane_f64(x, l); }
 __device__ __forceinline__ float c32_lane(float x, int l) {
@@ -179,8 +206,8 @@ pf_c32_kernel(BatchDesc d) {
                 const int pi = sw ? (p >> 1) : (i < 15 ? i + 1 : i);   // register of row p (> i)
                 const T rowp = A.get(pi);                  // row p lives in half p & 1
                 const T rowk1 = A.get(i);                  // row k+1 lives in the odd half
-                const T rowp_x = c32_xor32(rowp);          // row p seen from the other half
-                const T rowk1_x = c32_xor32(rowk1);        // row k+1 seen from the other half
+                T rowp_x, rowk1_x;                         // row p seen from the odd half, row k+1
+                c32_swap(rowk1, rowp, rowp_x, rowk1_x);    // seen from the even half
                 const bool podd = p & 1;
                 // row p's register takes row k+1's values (odd half when p is
                 // odd, else the even half through the exchange)
