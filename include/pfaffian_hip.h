@@ -182,8 +182,10 @@ int pfh_pfaffian_cpu_f64(double* A, int32_t n, double* out);
 #define PFH_TUNE_PANEL_SPIN 9       /* blocked panel: reads of the candidate records a poll makes before the
                                        panel sets its error flag (output NaN), 1 .. 4194304 (the default);
                                        0 sets it at once.  For tests of that path, not a tuning choice */
-#define PFH_TUNE_PANEL_XCD 10      /* blocked panel exchange: 1 through one XCD's L2 when the panel's blocks
-                                       fit one XCD, 0 device-wide; built-in: 1 */
+#define PFH_TUNE_PANEL_XCD 10      /* blocked panel exchange: 2 one participant per wave through one XCD's L2,
+                                       1 one per block of 256 slots through one XCD's L2, 0 one per block
+                                       device-wide (a form whose participants do not fit one XCD falls back to
+                                       the next); built-in: 2 */
 #define PFH_TUNE_MIXED_SLICES 11   /* PFH_METHOD_MIXED: 7-bit digits per value of the split-precision trailing
                                        update, 5 .. 9 */
 #define PFH_TUNE_BAT_NB_128 12     /* f64 batched blocked kernel, 64 < n <= 128: pairs per panel, 6, 8 or 12 */

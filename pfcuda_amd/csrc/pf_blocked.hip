@@ -74,6 +74,7 @@ static int xcd_tile_blocks(int ntile) {
 }
 constexpr int BC_MAXG = 64;        // cooperative panel: blocks at most (one wave polls and reduces their records)
 constexpr int BC_WAVES = BC_T / 64;
+constexpr int WP_MAXG = 256;       // per-wave panel: participants at most (a poll lane reads up to four records)
 // candidate records (described with bc_poll): the top bit carries the
 // epoch's parity; "no candidate" is NaN bits in A and index 2^31 - 1, slot -1
 // in B
@@ -166,7 +167,7 @@ __device__ __forceinline__ void accumulate(BPState<T>* st, T piv, bool sw) {
 }
 
 template <typename T>
-__global__ void bp_init(BPArgs<T> a, unsigned long long* rec) {
+__global__ void bp_init(BPArgs<T> a, unsigned long long* rec, unsigned long long* wrec) {
     const int64_t n = a.n;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         a.perm[i] = (int)i;
@@ -182,6 +183,12 @@ __global__ void bp_init(BPArgs<T> a, unsigned long long* rec) {
             const unsigned long long pb = bc_par((buf == 0 ? 3 : buf) + 1);
             rec[2 * q] = BC_NONE_A | pb;
             rec[2 * q + 1] = BC_NONE_B | pb;
+        }
+        for (int q = threadIdx.x; q < 3 * WP_MAXG; q += blockDim.x) {   // the same for the per-wave panel's
+            const int buf = q / WP_MAXG;
+            const unsigned long long pb = bc_par((buf == 0 ? 3 : buf) + 1);
+            wrec[2 * q] = BC_NONE_A | pb;
+            wrec[2 * q + 1] = BC_NONE_B | pb;
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -436,6 +443,7 @@ template <typename T>
 struct BPCoop {
     unsigned long long* rec;   // [3][BC_MAXG][2]: records A, B by epoch % 3
     T* cand;                   // [3][BC_MAXG][2 nb]: a block's candidate entries (real types), by epoch % 3
+    unsigned long long* wrec;  // [3][WP_MAXG][2]: the per-wave panel's records, by epoch % 3
 };
 
 // Accesses to the data the panel's blocks exchange within a launch (panel
@@ -974,6 +982,323 @@ __global__ void __launch_bounds__(BC_T) bp_panel(BPArgs<T> a, BPCoop<T> co, int 
     if (keeper) { st->lm = lm; st->le = le; st->ph_re = ph_re; st->ph_im = ph_im; }
 }
 
+// ---------------------------------------------------------------------------
+// The panel with one participant per wave (the XCD-local exchange only).
+// Each wave owns 64 consecutive slots and does by itself what a block of
+// bp_panel does with its four: it polls every participant's record, reduces
+// them, gathers the pivot rows' panel entries into its own LDS, updates its
+// slots and publishes its own record (its 64 slots' argmax).  A workgroup is
+// one wave, so a pair step holds no workgroup barrier, and every load that
+// depends on the pivot is issued in one run right after the poll (the
+// gathered entries first, the stale rows of W after them), with nothing read
+// before the poll that the poll's own waits would hold up.  The arithmetic,
+// and its order, is bp_panel's: the two forms agree bit for bit.
+// ---------------------------------------------------------------------------
+// Flat index of entry (pr, pc) in the stored upper triangle, and the entry
+// from the value read there: the load and the sign / zero fix-up are split
+// so that the fix-up can sit after other work.
+__device__ __forceinline__ int64_t wraw_ix(int64_t n, int pr, int pc) {
+    const int a = pr < pc ? pr : pc, b = pr < pc ? pc : pr;
+    return (int64_t)a * n + b;
+}
+template <typename T>
+__device__ __forceinline__ T wfix(T v, int pr, int pc) {
+    return pr < pc ? v : (pr > pc ? -v : zero<T>());
+}
+
+// Qpan[q] of the panel's first 2 NB + 2 logical rows, kept in two registers
+// (lane q of q0 for q < 64, lane q - 64 of q1 above); q is wave-uniform.
+__device__ __forceinline__ int wp_q(int q0, int q1, int q) {
+    return q < 64 ? __builtin_amdgcn_readlane(q0, q) : __builtin_amdgcn_readlane(q1, q - 64);
+}
+__device__ __forceinline__ void wp_qset(int& q0, int& q1, int q, int v) {
+    const int lane = (int)threadIdx.x;
+    if (q < 64) q0 = lane == q ? v : q0;
+    else q1 = lane == q - 64 ? v : q1;
+}
+
+// Every participant's record of epoch e, polled and reduced by the whole
+// wave (lane l reads records l, l + 64, ..., NMR of them, every read issued
+// before any is tested): false when a record did not show up within the spin
+// limit or another participant gave up (err is set, the output becomes NaN);
+// else found = some participant has a finite candidate, with its logical
+// index p and slot sp (ties: lowest index).
+template <int NMR>
+__device__ __forceinline__ bool wp_poll_n(const unsigned long long* rec, unsigned long long pb, int G, int limit,
+                                          int* err, int& p, int& sp, bool& found) {
+    const int lane = (int)threadIdx.x;
+    unsigned long long wa[NMR], wb[NMR];
+    for (int reads = 0;; ++reads) {
+        if (reads >= limit) {
+            if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+#pragma unroll
+        for (int m = 0; m < NMR; ++m) {
+            const int g = min(lane + 64 * m, G - 1);
+            wa[m] = rec_ld<XM_XCD>(&rec[2 * g]);
+            wb[m] = rec_ld<XM_XCD>(&rec[2 * g + 1]);
+        }
+        unsigned long long bad = 0;
+#pragma unroll
+        for (int m = 0; m < NMR; ++m) bad |= (wa[m] ^ pb) | (wb[m] ^ pb);
+        // another participant that gave up has set err: its record may never come
+        if ((reads & 15) == 15 &&
+            __builtin_amdgcn_readfirstlane(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0)
+            return false;
+        if (__all((bad & BC_PAR) == 0)) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    double best = -1.0;
+    int bi = 0x7fffffff, bs = 0;
+#pragma unroll
+    for (int m = 0; m < NMR; ++m) {
+        const double mg = __longlong_as_double((long long)(wa[m] & ~BC_PAR));
+        const int idx = (int)((wb[m] >> 32) & 0x7fffffffull);
+        if (lane + 64 * m < G && mg == mg && bc_better(mg, idx, best, bi)) {
+            best = mg;
+            bi = idx;
+            bs = (int)(unsigned)(wb[m] & 0xffffffffull);
+        }
+    }
+    double b;
+    int ix;
+    wave_argmax(best, bi, b, ix);
+    found = b >= 0.0;
+    if (found) {
+        const unsigned long long hit = __ballot(best == b && bi == ix);
+        p = ix;
+        sp = __builtin_amdgcn_readlane(bs, (int)__builtin_ctzll(hit));
+    }
+    return true;
+}
+__device__ __forceinline__ bool wp_poll(const unsigned long long* rec_all, int e, int G, int limit, int* err,
+                                        int& p, int& sp, bool& found) {
+    const unsigned long long* rec = rec_all + (int64_t)(e % 3) * WP_MAXG * 2;
+    const unsigned long long pb = bc_par(e);
+    if (G <= 64) return wp_poll_n<1>(rec, pb, G, limit, err, p, sp, found);
+    if (G <= 128) return wp_poll_n<2>(rec, pb, G, limit, err, p, sp, found);
+    return wp_poll_n<4>(rec, pb, G, limit, err, p, sp, found);
+}
+
+// The wave's argmax of (best, logical index) over its slots, and its record
+// for epoch e once the wave's stores (its new panel entries) have landed.
+__device__ __forceinline__ void wp_publish(double best, int lg, int x, int e, int rank, unsigned long long* rec_all) {
+    const int lane = (int)threadIdx.x;
+    double b;
+    int ix;
+    wave_argmax(best, lg, b, ix);                                        // ties: lowest index
+    const bool found = b >= 0.0;
+    int slot = 0;
+    if (found) slot = __builtin_amdgcn_readlane(x, (int)__builtin_ctzll(__ballot(best == b && lg == ix)));
+    __builtin_amdgcn_s_waitcnt(0);                                       // this wave's stores have landed
+    if (lane == 0) {
+        const unsigned long long pb = bc_par(e);
+        unsigned long long wa = BC_NONE_A | pb, wb = BC_NONE_B | pb;
+        if (found) {
+            wa = (unsigned long long)__double_as_longlong(b) | pb;       // b >= +0: sign bit clear
+            wb = pb | ((unsigned long long)(unsigned)ix << 32) | (unsigned long long)(unsigned)slot;
+        }
+        unsigned long long* rec = rec_all + ((int64_t)(e % 3) * WP_MAXG + rank) * 2;
+        rec_st<XM_XCD>(&rec[0], wa);
+        rec_st<XM_XCD>(&rec[1], wb);
+    }
+}
+
+// Launched with 8 G workgroups of one wave; those on XCD 0 claim the
+// participant ranks 0 .. G-1 (as bp_panel's blocks do) and the rest return.
+template <typename T, int NB>
+__global__ void __launch_bounds__(64) bp_wpanel(BPArgs<T> a, BPCoop<T> co, int k0, int np, int epoch0) {
+    constexpr int XM = XM_XCD;
+    constexpr int NH = (NB + 15) / 16;                  // stages of 16 pair steps
+    constexpr int CH = is_cplx<T>::value ? 4 : 8;       // pairs per chunk of the corrections
+    static_assert(NB % 8 == 0 && NB <= 32, "panel widths 8, 16, 24, 32");
+    // entries at one slot, interleaved (U_j at 2j, V_j at 2j + 1): Bx[0] the
+    // winner's (the new row k+1), Bx[1] the pre-interchange row k+1's,
+    // Bx[2] row k+2's
+    __shared__ __align__(16) T Bx[3][2 * NB];
+    const int lane = (int)threadIdx.x;
+    const int G = (int)(gridDim.x >> 3);
+    if (__builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)) != 0) return;   // HW_REG_XCC_ID[3:0]
+    int r0 = 0;
+    if (lane == 0) r0 = atomicAdd(&a.claim[k0 / (2 * NB)], 1);
+    const int rank = __builtin_amdgcn_readfirstlane(r0);
+    if (rank >= G) return;
+    BPState<T>* st = a.st;
+    if (st->dead || st->err) return;            // an earlier panel gave up: the output is NaN already
+    const int64_t n = a.n;
+    const int x = k0 + rank * 64 + lane;        // own slot
+    const bool own = x < n;
+    const int xs = own ? x : k0;                // a valid slot for masked reads
+    T* PU = a.P;                                // U_j[x] at PU[j*n + x]
+    T* PV = a.P + (int64_t)NB * n;              // V_j[x] at PV[j*n + x]
+
+    double lm = 0.0, ph_re = 0.0, ph_im = 0.0;  // pivot log / phase, kept by participant 0 lane 0
+    int le = 0;
+    const bool keeper = rank == 0 && lane == 0;
+    if (keeper) { lm = st->lm; le = st->le; ph_re = st->ph_re; ph_im = st->ph_im; }
+
+    int q0 = k0 + lane, q1 = k0 + 64 + lane;    // Qpan: slot of logical k0 + q
+    int lg = x;                                 // logical index of the own slot
+    BPRegs<T, NB> Ur, Vr;                       // U_j, V_j at the own slot
+    bp_sfor<0, NB>([&](auto jc) {
+        constexpr int J = decltype(jc)::value;
+        Ur.template put<J>(zero<T>());
+        Vr.template put<J>(zero<T>());
+    });
+
+    // first pivot row of the panel: row k0 of W (nothing pending); epoch
+    // epoch0 + 1 + i carries the candidates of pair step i
+    {
+        double best = -1.0;
+        T r = zero<T>();
+        if (own && x > k0) {
+            r = a.W[(int64_t)k0 * n + x];
+            xb_st<XM>(&PV[x], r);
+            const double m = (double)mag2(r);
+            if (m == m) best = m;
+        }
+        Vr.template put<0>(r);
+        wp_publish(best, lg, x, epoch0 + 1, rank, co.wrec);
+    }
+
+    int quit = 0;   // 1: the panel is done (or the matrix); 2: leave at once (zero pivot, failed poll)
+    bp_sfor<0, NH>([&](auto hc) {
+        constexpr int H = decltype(hc)::value;
+        if (quit) return;
+        const int iend = min(np, 16 * H + 16);
+        for (int i = 16 * H; i < iend; ++i) {
+            const int il = i - 16 * H;
+            const int k = k0 + 2 * i;
+            const int e = epoch0 + 1 + i;
+            const bool has_next = i + 1 < np;       // row k+2 is this panel's next pivot row
+            const int sa = wp_q(q0, q1, k + 1 - k0);
+            const int sb = has_next ? wp_q(q0, q1, k + 2 - k0) : -1;
+            const int sbs = sb >= 0 ? sb : sa;
+
+            int p = k + 1, sp = sa;
+            bool wf = false;
+            if (!wp_poll(co.wrec, e, G, a.spin, &st->err, p, sp, wf)) { quit = 2; return; }
+            if (!wf) { p = k + 1; sp = sa; }                   // no finite candidate (NaN row)
+            const bool sw = p != k + 1;
+            // the interchange k+1 <-> p as seen from the own slot
+            if (x == sp) lg = k + 1;
+            else if (sw && x == sa) lg = p;
+            const int y1 = sp;
+            const int y2 = (p == k + 2) ? sa : sb;
+            const int y2s = has_next ? y2 : y1;
+
+            // ---- every load that depends on the pivot, in one run: the
+            //      entries U_j, V_j at the winner's slot and at the
+            //      pre-interchange slots of rows k+1 and k+2 (lane 2j + v),
+            //      then rows k+1 and k+2 of W at the own slots and entry
+            //      (k+1, k+2); each from a valid address, masked after
+            const int l = lane < 2 * NB ? lane : 2 * NB - 1;
+            const int jl = l >> 1;
+            const bool isv = l & 1;
+            const T* src = (isv ? PV : PU) + (int64_t)jl * n;
+            const T g0 = xb_ld<XM>(&src[sp]);
+            const T g1 = xb_ld<XM>(&src[sa]);
+            const T g2 = xb_ld<XM>(&src[sbs]);
+            const T w1r = a.W[wraw_ix(n, y1, xs)];
+            const T w2r = a.W[wraw_ix(n, y2s, xs)];
+            const T w12r = a.W[wraw_ix(n, y1, y2s)];
+            const bool ok = isv ? jl <= i : jl < i;             // U_j for j < i, V_j for j <= i
+            __builtin_amdgcn_wave_barrier();
+            if (lane < 2 * NB) {
+                Bx[0][lane] = psel(ok, g0, zero<T>());
+                Bx[1][lane] = psel(ok, g1, zero<T>());
+                Bx[2][lane] = psel(ok && sb >= 0, g2, zero<T>());
+            }
+            __builtin_amdgcn_wave_barrier();
+            const T* b1 = Bx[0];                               // the pivot's entries (row k+1 when no winner: sp = sa)
+            const T piv = b1[2 * i + 1];                       // V_i there
+            if (is_zero(piv)) {                                // same decision in every participant
+                if (keeper) st->dead = 1;
+                quit = 2;
+                return;
+            }
+            if (keeper) accumulate(lm, le, ph_re, ph_im, piv, sw);
+            if (sw) {
+                wp_qset(q0, q1, k + 1 - k0, sp);
+                if ((unsigned)(p - k0) < (unsigned)(2 * NB + 2)) wp_qset(q0, q1, p - k0, sa);
+            }
+            const T* b2 = Bx[(p == k + 2) ? 1 : 2];
+            if (k + 2 >= n) { quit = 1; return; }              // last pair of the matrix
+
+            const T inv = recip(piv);
+            // corrections of the pairs j < i, in chunks, as in bp_panel
+            T r1 = zero<T>(), rl = zero<T>(), rk = zero<T>();
+            T r1m = zero<T>(), rlm = zero<T>(), rkm = zero<T>();
+            const BCPair<T>* P1 = reinterpret_cast<const BCPair<T>*>(b1);
+            const BCPair<T>* P2 = reinterpret_cast<const BCPair<T>*>(b2);
+            bp_sfor<0, NB / CH>([&](auto cc) {
+                constexpr int C = decltype(cc)::value;
+                if (C * CH >= i) return;
+                BCPair<T> c1[CH], c2[CH];
+                bp_sfor<0, CH>([&](auto qc) {
+                    constexpr int Q = decltype(qc)::value;
+                    c1[Q] = P1[C * CH + Q];
+                    c2[Q] = P2[C * CH + Q];
+                });
+                bp_sfor<0, CH>([&](auto qc) {
+                    constexpr int Q = decltype(qc)::value;
+                    constexpr int J = C * CH + Q;
+                    const T uj = Ur.template at<J>(), vj = Vr.template at<J>();
+                    const T b1u = c1[Q].u, b1v = c1[Q].v, b2u = c2[Q].u, b2v = c2[Q].v;
+                    if constexpr (is_cplx<T>::value) {
+                        cfma(r1, b1u, vj); cfma(r1m, b1v, uj);
+                        cfma(rl, b2u, vj); cfma(rlm, b2v, uj);
+                        cfma(rk, b1u, b2v); cfma(rkm, b1v, b2u);
+                    } else {
+                        r1 = r1 + b1u * vj - b1v * uj;
+                        rl = rl + b2u * vj - b2v * uj;
+                        rk = rk + b1u * b2v - b1v * b2u;
+                    }
+                });
+            });
+            if constexpr (is_cplx<T>::value) {
+                r1 = r1 - r1m;
+                rl = rl - rlm;
+                rk = rk - rkm;
+            }
+            const bool act1 = own && lg >= k + 2;
+            T u = zero<T>();
+            if (act1) {
+                u = (wfix(w1r, y1, xs) + r1) * inv;
+                xb_st<XM>(&PU[(int64_t)i * n + x], u);
+            }
+            Ur.template set<H>(il, u);
+            if (!has_next) { quit = 1; return; }
+
+            // ---- row k+2 (the next pivot row) at the own slots -> V_{i+1}
+            const T u_kl = (wfix(w12r, y1, y2s) + rk) * inv;   // U_i at row k+2
+            const T v_kl = b2[2 * i + 1];                      // V_i at row k+2
+            const T vi = Vr.template get<H>(il);
+            double best = -1.0;
+            T r = zero<T>();
+            if (own && lg >= k + 3) {
+                r = wfix(w2r, y2, xs) + rl + u_kl * vi - v_kl * u;
+                xb_st<XM>(&PV[(int64_t)(i + 1) * n + x], r);
+                const double m = (double)mag2(r);
+                if (m == m) best = m;
+            }
+            if constexpr (H + 1 < NH) {
+                if (il == 15) Vr.template put<16 * (H + 1)>(r);
+                else Vr.template set<H>(min(il + 1, 15), r);
+            } else {
+                constexpr int LH = BPRegHalf<double, NB>::template len<H>();
+                Vr.template set<H>(min(il + 1, LH - 1), r);    // i + 1 < np <= NB
+            }
+            wp_publish(best, lg, x, e + 1, rank, co.wrec);
+        }
+    });
+    if (quit == 2) return;
+
+    if (own) a.perm[lg] = x;   // logical -> slot, for the trailing update
+    if (keeper) { st->lm = lm; st->le = le; st->ph_re = ph_re; st->ph_im = ph_im; }
+}
+
 // Trailing update over upper tiles of the block [lo, n)^2:
 // Wn[r][c] = Wo[perm r][perm c] + sum_{j<np} (U_j[s r] V_j[s c] - V_j[s r] U_j[s c]),  r < c,
 // with s = perm when the panel vectors are stored by slot (pslot), else identity.
@@ -1466,7 +1791,7 @@ static size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 
 template <typename T>
 struct BPLayout {
-    size_t w0, w1, p, r, perm, pv, pi, st, claim, rec, cand, mxd, mxe, total;
+    size_t w0, w1, p, r, perm, pv, pi, st, claim, rec, wrec, cand, mxd, mxe, total;
 };
 
 template <typename T>
@@ -1484,6 +1809,7 @@ static BPLayout<T> bp_layout(int64_t n, int nb, bool mixed = false) {
     L.st = o; o = al(o + sizeof(BPState<T>));
     L.claim = o; o = al(o + sizeof(int) * (size_t)(n / (2 * nb) + 2));
     L.rec = o; o = al(o + sizeof(unsigned long long) * 3 * BC_MAXG * 2);
+    L.wrec = o; o = al(o + sizeof(unsigned long long) * 3 * WP_MAXG * 2);
     L.cand = o; o = al(o + sizeof(T) * 3 * BC_MAXG * 2 * nb);
     L.mxd = L.mxe = o;
     if (mixed) {   // split-precision digits (up to 8 per value) and slot scales
@@ -1536,20 +1862,35 @@ static bool coop_enabled() {
 // 28.6, gradient N = 2048 5.82 / 6.59; c128 N = 4096 12.4 / 14.4.  (With
 // the panel vectors in LDS, real types had lost: f64 N = 4096 12.1 / 9.9,
 // the strided reads of W's columns, one cache line per entry, all going
-// through one XCD's L2.)  PFH_PANEL_XCD=0/1 or PFH_TUNE_PANEL_XCD set it
-// for every type.  Read per call.
-static int panel_xcd_max_grid() {
+// through one XCD's L2.)  The panel form: 2 one participant per wave
+// (bp_wpanel, XCD-local; the default), 1 one per block (bp_panel,
+// XCD-local), 0 one per block device-wide; PFH_PANEL_XCD=0/1/2 or
+// PFH_TUNE_PANEL_XCD set it for every type, read per call.  A form whose
+// participants do not fit leaves the panel to the next one down.
+static int panel_form() {
     const int tv = tune_value(PFH_TUNE_PANEL_XCD);
-    bool on = true;
+    int f = 2;
     if (tv >= 0) {
-        on = tv == 1;
+        f = tv;
     } else {
         const char* e = std::getenv("PFH_PANEL_XCD");
-        if (e && (e[0] == '0' || e[0] == '1')) on = e[0] == '1';
+        if (e && e[0] >= '0' && e[0] <= '2') f = e[0] - '0';
     }
-    if (!on) return 0;
-    const int cus = device_cu_count();
-    return (cus % 8 == 0) ? cus / 8 : 0;
+    return device_cu_count() % 8 == 0 ? f : 0;
+}
+static int panel_xcd_max_grid() { return panel_form() >= 1 ? device_cu_count() / 8 : 0; }
+
+// Waves of bp_wpanel<T, NB> that XCD 0 holds at once (its CUs times the
+// kernel's workgroups per CU), 0 when the query fails.
+template <typename T, int NB>
+static int wpanel_capacity() {
+    static DevCache cache;   // capacity + 1
+    return cache.get([] {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bp_wpanel<T, NB>, 64, 0) != hipSuccess)
+            per_cu = 0;
+        return per_cu * (device_cu_count() / 8) + 1;
+    }) - 1;
 }
 
 template <typename T, int NB, bool LOCAL>
@@ -1568,11 +1909,25 @@ static hipError_t launch_panel_form(const BPArgs<T>& a, const BPCoop<T>& co, int
     return hipGetLastError();
 }
 
+// One panel launch over the columns [k0, n): the per-wave form while its
+// participants fit XCD 0 together, else the per-block forms.  Each form
+// keeps its own epoch count (its records' parities follow it), advanced by
+// the panel's np.
 template <typename T, int NB>
-static hipError_t launch_panel(const BPArgs<T>& a, const BPCoop<T>& co, int k0, int np, int epoch0, int G,
+static hipError_t launch_panel(const BPArgs<T>& a, const BPCoop<T>& co, int k0, int np, int& epoch, int& wepoch,
                                hipStream_t s) {
-    if (G <= panel_xcd_max_grid()) return launch_panel_form<T, NB, true>(a, co, k0, np, epoch0, G, s);
-    return launch_panel_form<T, NB, false>(a, co, k0, np, epoch0, G, s);
+    const int64_t m = a.n - k0;
+    const int Gw = (int)((m + 63) / 64);
+    if (panel_form() == 2 && Gw <= WP_MAXG && Gw <= wpanel_capacity<T, NB>()) {
+        hipLaunchKernelGGL((bp_wpanel<T, NB>), dim3(8 * Gw), dim3(64), 0, s, a, co, k0, np, wepoch);
+        wepoch += np;
+        return hipGetLastError();
+    }
+    const int G = (int)((m + BC_T - 1) / BC_T);
+    const hipError_t e = G <= panel_xcd_max_grid() ? launch_panel_form<T, NB, true>(a, co, k0, np, epoch, G, s)
+                                                   : launch_panel_form<T, NB, false>(a, co, k0, np, epoch, G, s);
+    epoch += np;
+    return e;
 }
 
 int blocked_take_status(hipStream_t s, int* timed_out) {
@@ -1624,6 +1979,7 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
     a.spin = tune_value(PFH_TUNE_PANEL_SPIN) >= 0 ? tune_value(PFH_TUNE_PANEL_SPIN) : (1 << 22);
     BPCoop<T> co{};
     co.rec = (unsigned long long*)(base + L.rec);
+    co.wrec = (unsigned long long*)(base + L.wrec);
     co.cand = (T*)(base + L.cand);
     const bool coop = coop_enabled() &&
                       (std::is_same<T, c128>::value ? (nb == 8 || nb == 16)
@@ -1631,9 +1987,9 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
     hipError_t e = hipMemcpyAsync(W[0], A, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return (int)e;
     a.W = W[0];
-    hipLaunchKernelGGL(bp_init<T>, dim3(64), dim3(256), 0, s, a, co.rec);
+    hipLaunchKernelGGL(bp_init<T>, dim3(64), dim3(256), 0, s, a, co.rec, co.wrec);
     int cur = 0;
-    int epoch = 0;   // grid barriers passed so far (cooperative panels)
+    int epoch = 0, wepoch = 0;   // epochs passed so far: per-block and per-wave panels
     auto blocks = [&](int64_t c_lo) { return (unsigned)((n - c_lo + BP_THREADS - 1) / BP_THREADS); };
     for (int64_t k0 = 0; k0 + 1 < n; k0 += 2 * nb) {
         const int np = (int)((n - k0) / 2 < nb ? (n - k0) / 2 : nb);
@@ -1642,17 +1998,16 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
         const bool panel_coop = coop && G <= BC_MAXG;
         if (panel_coop) {
             if constexpr (std::is_same<T, c128>::value) {
-                e = (nb == 8) ? launch_panel<T, 8>(a, co, (int)k0, np, epoch, G, s)
-                              : launch_panel<T, 16>(a, co, (int)k0, np, epoch, G, s);
+                e = (nb == 8) ? launch_panel<T, 8>(a, co, (int)k0, np, epoch, wepoch, s)
+                              : launch_panel<T, 16>(a, co, (int)k0, np, epoch, wepoch, s);
             } else if constexpr (std::is_same<T, double>::value) {
-                e = (nb == 16) ? launch_panel<T, 16>(a, co, (int)k0, np, epoch, G, s)
-                  : (nb == 24) ? launch_panel<T, 24>(a, co, (int)k0, np, epoch, G, s)
-                               : launch_panel<T, 32>(a, co, (int)k0, np, epoch, G, s);
+                e = (nb == 16) ? launch_panel<T, 16>(a, co, (int)k0, np, epoch, wepoch, s)
+                  : (nb == 24) ? launch_panel<T, 24>(a, co, (int)k0, np, epoch, wepoch, s)
+                               : launch_panel<T, 32>(a, co, (int)k0, np, epoch, wepoch, s);
             } else {
-                e = launch_panel<T, 32>(a, co, (int)k0, np, epoch, G, s);
+                e = launch_panel<T, 32>(a, co, (int)k0, np, epoch, wepoch, s);
             }
             if (e != hipSuccess) return (int)e;
-            epoch += np;   // one barrier for the first row, one per pair step but the last
         } else {
         // first pivot row of the panel
         hipLaunchKernelGGL((bp_row<T, false, true>), dim3(blocks(k0 + 1)), dim3(BP_THREADS), 0, s, a, (int)k0, 0,
@@ -2216,6 +2571,7 @@ int blocked_grad(const T* A, int64_t n, void* ws, double* log_out, T* phase_out,
     a.spin = tune_value(PFH_TUNE_PANEL_SPIN) >= 0 ? tune_value(PFH_TUNE_PANEL_SPIN) : (1 << 22);
     BPCoop<T> co{};
     co.rec = (unsigned long long*)(base + L.bp.rec);
+    co.wrec = (unsigned long long*)(base + L.bp.wrec);
     co.cand = (T*)(base + L.bp.cand);
     T* S = (T*)(base + L.s);
     T* Z = (T*)(base + L.z);
@@ -2225,18 +2581,17 @@ int blocked_grad(const T* A, int64_t n, void* ws, double* log_out, T* phase_out,
     hipError_t e = hipMemcpyAsync(W[0], A, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return (int)e;
     a.W = W[0];
-    hipLaunchKernelGGL(bp_init<T>, dim3(64), dim3(256), 0, s, a, co.rec);
+    hipLaunchKernelGGL(bp_init<T>, dim3(64), dim3(256), 0, s, a, co.rec, co.wrec);
     hipLaunchKernelGGL(bp_reset_perm<T>, dim3(64), dim3(256), 0, s, gp[0], (int64_t)0, n);
     const int ntile = (int)((n + BP_TILE - 1) / BP_TILE);
     const int tiles = xcd_tile_blocks(ntile);
-    int cur = 0, gc = 0, epoch = 0;
+    int cur = 0, gc = 0, epoch = 0, wepoch = 0;
     for (int64_t k0 = 0; k0 + 1 < n; k0 += 2 * NB) {
         const int np = (int)((n - k0) / 2 < NB ? (n - k0) / 2 : NB);
         const int m = 2 * np;
         a.W = W[cur];
-        e = launch_panel<T, NB>(a, co, (int)k0, np, epoch, (int)((n - k0 + BC_T - 1) / BC_T), s);
+        e = launch_panel<T, NB>(a, co, (int)k0, np, epoch, wepoch, s);
         if (e != hipSuccess) return (int)e;
-        epoch += np;
         hipLaunchKernelGGL((gj_inv<T, NB>), dim3(1), dim3(1024), 0, s, W[cur], a.perm, n, (int)k0, np, a.st, Ai);
         hipLaunchKernelGGL((gj_rows<T, NB>), dim3((unsigned)((n + GJ_CB - 1) / GJ_CB)), dim3(256), 0, s, W[cur], a.perm, n,
                            (int)k0, np, a.st, Ai, S, Z, gp[gc], gp[gc ^ 1]);

@@ -39,8 +39,8 @@ bool tune_value_ok(int key, int v) {
         case PFH_TUNE_TRAIL_C128:
         case PFH_TUNE_TRAIL_SWEEP:
         case PFH_TUNE_GRAD_BLOCKED:
-        case PFH_TUNE_PANEL_XCD:
         case PFH_TUNE_SMALL_KERNEL: return v == 0 || v == 1;
+        case PFH_TUNE_PANEL_XCD: return v >= 0 && v <= 2;
         case PFH_TUNE_PANEL_LAUNCH: return v == 0;   // the plain launch is the only form
         case PFH_TUNE_PANEL_SPIN: return v >= 0 && v <= (1 << 22);
         case PFH_TUNE_MIXED_SLICES: return v >= 5 && v <= 9;

@@ -544,11 +544,11 @@ def test_trailing_forms_bitwise_equal(pf, cuda, monkeypatch, dtype):
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.complex128, torch.float32])
 def test_panel_exchange_forms_bitwise_equal(pf, cuda, dtype):
-    # The panel's blocks exchange candidates through one XCD's L2 (the
-    # default while the panel fits one XCD) or device-wide; the arithmetic
-    # and its order are the same, so the two agree bit for bit (blocked
-    # factorization at n = 1000, whose panels cross several blocks, and the
-    # blocked sweep at n = 600).
+    # The panel's participants (one per wave, the default, or one per block
+    # of 256 slots) exchange candidates through one XCD's L2, or (blocks)
+    # device-wide; the arithmetic and its order are the same, so the three
+    # forms agree bit for bit (blocked factorization at n = 1000, whose
+    # panels cross several blocks, and the blocked sweep at n = 600).
     from pfcuda_amd import _lib
     from pfcuda_amd.adjoint import slog_grad
     rng = np.random.default_rng(62)
@@ -558,7 +558,7 @@ def test_panel_exchange_forms_bitwise_equal(pf, cuda, dtype):
     key = _lib.TUNE_KEYS["panel_xcd"]
     res = {}
     try:
-        for form in (1, 0):
+        for form in (2, 1, 0):
             _lib.check(_lib.lib.pfh_set_tuning(key, form), "panel_xcd")
             la, s = pf.slog_pfaffian(At, method=pf.PFH_METHOD_BLOCKED)
             G = slog_grad(Bt) if dtype != torch.float32 else torch.zeros(1)
@@ -566,11 +566,12 @@ def test_panel_exchange_forms_bitwise_equal(pf, cuda, dtype):
         pf.check_status()
     finally:
         _lib.lib.pfh_set_tuning(key, -1)
-    assert res[1][0] == res[0][0] and res[1][1] == res[0][1]
-    assert np.array_equal(res[1][2], res[0][2])
+    for form in (1, 0):
+        assert res[2][0] == res[form][0] and res[2][1] == res[form][1], form
+        assert np.array_equal(res[2][2], res[form][2]), form
     if dtype != torch.float32:
         l0, p0 = oracle.slog(A)
-        assert abs(res[1][0] - l0) < 1e-9 and abs(res[1][1] - p0) < 1e-9
+        assert abs(res[2][0] - l0) < 1e-9 and abs(res[2][1] - p0) < 1e-9
 
 
 def test_grad_n2048_residual(pf, cuda):
