@@ -1200,9 +1200,15 @@ __global__ void __launch_bounds__(64) bp_wpanel(BPArgs<T> a, BPCoop<T> co, int k
             const T g0 = xb_ld<XM>(&src[sp]);
             const T g1 = xb_ld<XM>(&src[sa]);
             const T g2 = xb_ld<XM>(&src[sbs]);
+            // (the column of entry (k+1, k+2) passes through a vector register:
+            // from a uniform address the compiler would move the value to a
+            // scalar register at once, and the wait for the entries above
+            // would then hold for this HBM read too)
+            int y2v;
+            asm volatile("v_mov_b32 %0, %1" : "=v"(y2v) : "v"(y2s));
             const T w1r = a.W[wraw_ix(n, y1, xs)];
             const T w2r = a.W[wraw_ix(n, y2s, xs)];
-            const T w12r = a.W[wraw_ix(n, y1, y2s)];
+            const T w12r = a.W[wraw_ix(n, y1, y2v)];
             const bool ok = isv ? jl <= i : jl < i;             // U_j for j < i, V_j for j <= i
             __builtin_amdgcn_wave_barrier();
             if (lane < 2 * NB) {
