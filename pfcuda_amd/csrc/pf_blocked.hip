@@ -1268,12 +1268,14 @@ __global__ void __launch_bounds__(64) bp_wpanel(BPArgs<T> a, BPCoop<T> co, int k
                 rl = rl - rlm;
                 rk = rk - rkm;
             }
+            // every lane stores U_i and V_{i+1} (zero at the slots not
+            // active, which are never read back; lanes past n at slot k0):
+            // a store under a branch would make the compiler wait at the
+            // branch's end for it and for the reads it consumes
             const bool act1 = own && lg >= k + 2;
             T u = zero<T>();
-            if (act1) {
-                u = (wfix(w1r, y1, xs) + r1) * inv;
-                xb_st<XM>(&PU[(int64_t)i * n + x], u);
-            }
+            if (act1) u = (wfix(w1r, y1, xs) + r1) * inv;
+            xb_st<XM>(&PU[(int64_t)i * n + xs], u);
             Ur.template set<H>(il, u);
             if (!has_next) { quit = 1; return; }
 
@@ -1281,14 +1283,12 @@ __global__ void __launch_bounds__(64) bp_wpanel(BPArgs<T> a, BPCoop<T> co, int k
             const T u_kl = (wfix(w12r, y1, y2s) + rk) * inv;   // U_i at row k+2
             const T v_kl = b2[2 * i + 1];                      // V_i at row k+2
             const T vi = Vr.template get<H>(il);
-            double best = -1.0;
+            const bool act2 = own && lg >= k + 3;
             T r = zero<T>();
-            if (own && lg >= k + 3) {
-                r = wfix(w2r, y2, xs) + rl + u_kl * vi - v_kl * u;
-                xb_st<XM>(&PV[(int64_t)(i + 1) * n + x], r);
-                const double m = (double)mag2(r);
-                if (m == m) best = m;
-            }
+            if (act2) r = wfix(w2r, y2, xs) + rl + u_kl * vi - v_kl * u;
+            xb_st<XM>(&PV[(int64_t)(i + 1) * n + xs], r);
+            const double m = (double)mag2(r);
+            const double best = (act2 && m == m) ? m : -1.0;
             if constexpr (H + 1 < NH) {
                 if (il == 15) Vr.template put<16 * (H + 1)>(r);
                 else Vr.template set<H>(min(il + 1, 15), r);
