@@ -2154,6 +2154,10 @@ __global__ void __launch_bounds__(1024) gj_inv(const T* __restrict__ W, const in
         const T w = w_at(W, n, pg[min(r, m - 1)], pg[min(c, m - 1)]);
         v[i] = (r < m && c < m) ? w : zero<T>();
     }
+    // a full group (m = M, every group but possibly a matrix's last) has no
+    // padding entries to keep at zero: its steps skip that select
+    auto sweep = [&](auto fullc) {
+    constexpr bool FULL = decltype(fullc)::value;
     bp_sfor<0, NB>([&](auto jc) {
         constexpr int J = decltype(jc)::value, K = 2 * J, PAR = J & 1;
         if (J >= np) return;
@@ -2184,7 +2188,7 @@ __global__ void __launch_bounds__(1024) gj_inv(const T* __restrict__ W, const in
                     const T colv = psel(c == K, -(yrr * inv), xrr * inv);
                     u = psel(c == K || c == K + 1, colv, gen);
                 }
-                v[I] = psel(r < m && c < m, u, v[I]);
+                v[I] = FULL ? u : psel(r < m && c < m, u, v[I]);
             });
         } else {
 #pragma unroll
@@ -2194,10 +2198,13 @@ __global__ void __launch_bounds__(1024) gj_inv(const T* __restrict__ W, const in
                 const T rowv = psel(r == K, yc[i] * inv, -(xc[i] * inv));
                 const T pp = psel(r == c, zero<T>(), psel(r == K, ninv, inv));
                 const T u = psel(pc, pp, rowv);
-                v[i] = psel(r < m && c < m, u, v[i]);
+                v[i] = FULL ? u : psel(r < m && c < m, u, v[i]);
             }
         }
     });
+    };
+    if (m == M) sweep(std::true_type{});
+    else sweep(std::false_type{});
 #pragma unroll
     for (int i = 0; i < EPT; ++i) Ai[r * M + cq + TPR * i] = v[i];
 }
