@@ -2209,14 +2209,19 @@ __global__ void __launch_bounds__(1024) gj_inv(const T* __restrict__ W, const in
     for (int i = 0; i < EPT; ++i) Ai[r * M + cq + TPR * i] = v[i];
 }
 
-constexpr int GJ_CB = 16;   // gj_rows: columns per block (16 row groups of 256 / 16 threads)
+#ifndef GJ_CB64
+#define GJ_CB64 8
+#endif
+// gj_rows: columns per block for a group of M rows (256 / CB row groups)
+template <int M>
+constexpr int gj_cb() { return M >= 64 ? GJ_CB64 : 8; }
 
 template <typename T, int NB>
 __global__ void __launch_bounds__(256) gj_rows(const T* __restrict__ W, const int* __restrict__ perm, int64_t n,
                                                int k0, int np, const BPState<T>* st, const T* __restrict__ Ai,
                                                T* __restrict__ S, T* __restrict__ Z, const int* __restrict__ gp_old,
                                                int* __restrict__ gp_new) {
-    constexpr int M = 2 * NB, CB = GJ_CB, HG = 256 / CB, HPT = M / HG;
+    constexpr int M = 2 * NB, CB = gj_cb<M>(), HG = 256 / CB, HPT = M / HG;
     static_assert(256 % CB == 0 && M % HG == 0, "row groups tile the block");
     __shared__ T As[M][M + 1];
     __shared__ T Sc[M][CB + 1];
@@ -2612,7 +2617,7 @@ int blocked_grad(const T* A, int64_t n, void* ws, double* log_out, T* phase_out,
         e = launch_panel<T, NB>(a, co, (int)k0, np, epoch, wepoch, s);
         if (e != hipSuccess) return (int)e;
         hipLaunchKernelGGL((gj_inv<T, NB>), dim3(1), dim3(1024), 0, s, W[cur], a.perm, n, (int)k0, np, a.st, Ai);
-        hipLaunchKernelGGL((gj_rows<T, NB>), dim3((unsigned)((n + GJ_CB - 1) / GJ_CB)), dim3(256), 0, s, W[cur], a.perm, n,
+        hipLaunchKernelGGL((gj_rows<T, NB>), dim3((unsigned)((n + gj_cb<2 * NB>() - 1) / gj_cb<2 * NB>())), dim3(256), 0, s, W[cur], a.perm, n,
                            (int)k0, np, a.st, Ai, S, Z, gp[gc], gp[gc ^ 1]);
         if constexpr (std::is_same<T, double>::value || std::is_same<T, c128>::value) {
             if (trail_staged(PFH_TUNE_TRAIL_SWEEP, true)) {
