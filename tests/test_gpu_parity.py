@@ -574,6 +574,33 @@ def test_panel_exchange_forms_bitwise_equal(pf, cuda, dtype):
         assert abs(res[2][0] - l0) < 1e-9 and abs(res[2][1] - p0) < 1e-9
 
 
+def test_panel_forms_switch_within_one_solve(pf, cuda):
+    # complex128 at n = 8320: the per-wave panel holds at most 128 waves
+    # (8192 columns), so the first panels run in the per-block form and the
+    # later ones per wave, each form with its own epoch count; the solve
+    # equals, bit for bit, the one with every panel in the per-block forms
+    # (PFH_PANEL_XCD = 1: XCD-local while the blocks fit one XCD, else
+    # device-wide), and agrees with the unblocked path.
+    from pfcuda_amd import _lib
+    rng = np.random.default_rng(83)
+    n = 8320
+    A = oracle.skew_gaussian(rng, n, complex_=True)
+    At = torch.from_numpy(A).to(cuda)
+    key = _lib.TUNE_KEYS["panel_xcd"]
+    res = {}
+    try:
+        for form in (2, 1):
+            _lib.check(_lib.lib.pfh_set_tuning(key, form), "panel_xcd")
+            la, ph = pf.slog_pfaffian(At, method=pf.PFH_METHOD_BLOCKED)
+            res[form] = (float(la), complex(ph.item()))
+        pf.check_status()
+    finally:
+        _lib.lib.pfh_set_tuning(key, -1)
+    assert res[2] == res[1]
+    lu, pu = pf.slog_pfaffian(At, method=pf.PFH_METHOD_UNBLOCKED)
+    assert abs(res[2][0] - float(lu)) < 1e-8 and abs(res[2][1] - complex(pu.item())) < 1e-8
+
+
 def test_grad_n2048_residual(pf, cuda):
     # C5 at full size: A G^T = I/2 (G = 1/2 A^-T), and the slog against the oracle
     from pfcuda_amd.adjoint import slog_grad
