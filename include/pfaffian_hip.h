@@ -200,7 +200,8 @@ int pfh_pfaffian_cpu_f64(double* A, int32_t n, double* out);
                                       batched kernels take (64, 96, ..., 256); larger ones run blocked */
 #define PFH_TUNE_SMALL_KERNEL 21   /* pfaffian / slog, n <= 32: 1 one wave per matrix updating the live rows (the
                                       default), 0 the column-per-lane kernel that updates the whole square */
-#define PFH_TUNE_NKEYS 22
+#define PFH_TUNE_GRAD_NB 22         /* blocked gradient sweep, float64: pairs per group, 16 (the default) or 32 */
+#define PFH_TUNE_NKEYS 23
 
 /* Sets a tuning key (value -1: the built-in choice); PFH_EINVAL for an
  * unknown key or a value outside the key's list.  Takes effect for calls

@@ -25,7 +25,7 @@ TUNE_KEYS = {
     "panel_spin": 9, "panel_xcd": 10, "mixed_slices": 11,
     "bat_nb_128": 12, "bat_nb_192": 13, "bat_nb_256": 14,
     "bat_nb_c128_128": 15, "bat_nb_c128_192": 16, "bat_nb_c128_256": 17,
-    "blocked_nb": 18, "blocked_nb_c128": 19, "switch_n": 20, "small_kernel": 21,
+    "blocked_nb": 18, "blocked_nb_c128": 19, "switch_n": 20, "small_kernel": 21, "grad_nb": 22,
 }
 
 _vp = ctypes.c_void_p

@@ -39,7 +39,7 @@ CANDIDATES = {
     # panel widths (pairs per panel) and the single-matrix switch point
     "bat_nb_128": [6, 8, 12], "bat_nb_192": [6, 8, 12], "bat_nb_256": [8, 10, 12],
     "bat_nb_c128_128": [4, 6, 8], "bat_nb_c128_192": [4, 6, 8], "bat_nb_c128_256": [6, 8],
-    "blocked_nb": [16, 24, 32], "blocked_nb_c128": [8, 16],
+    "blocked_nb": [16, 24, 32], "blocked_nb_c128": [8, 16], "grad_nb": [16, 32],
     "switch_n": [96, 128, 160, 192, 224, 256],
     "small_kernel": [0, 1],
 }
@@ -143,7 +143,7 @@ def _single_workload(torch, name, n, dev):
     gen.manual_seed(4321 + n)
     cplx = name in ("trail_c128", "blocked_nb_c128")
     A = _skew(torch, (n, n), cplx, gen, dev)
-    if name in ("trail_sweep", "grad_blocked"):
+    if name in ("trail_sweep", "grad_blocked", "grad_nb"):
         def run():
             G, la, ph = slog_grad(A, return_slog=True)
             return la, ph, G

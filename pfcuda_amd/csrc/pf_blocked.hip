@@ -1843,7 +1843,15 @@ int blocked_panel_width(int dtype) {
     }
     return 32;
 }
-static int grad_panel_width(int dtype) { return dtype == DT_C128 ? 16 : 32; }
+// Pair steps per group of the blocked block sweep: complex128 16; float64
+// 16 (PFH_TUNE_GRAD_NB may set 32: measured at n = 2048, 16 pairs 3.46 ms
+// against 3.55 at 32 -- shorter pair steps and a 32 x 32 inversion per group
+// outweigh the twice as many trailing passes); other types 32.
+static int grad_panel_width(int dtype) {
+    if (dtype == DT_C128) return 16;
+    if (dtype == DT_F64) return tune_value(PFH_TUNE_GRAD_NB) == 32 ? 32 : 16;
+    return 32;
+}
 
 // The panel kernel runs on a device with at least BC_MAXG CUs (so that a
 // panel's blocks, one per CU, are resident together), unless
@@ -2575,9 +2583,8 @@ size_t blocked_grad_workspace_bytes(int dtype, int64_t n) {
     return 0;
 }
 
-template <typename T>
-int blocked_grad(const T* A, int64_t n, void* ws, double* log_out, T* phase_out, T* G, hipStream_t s) {
-    constexpr int NB = std::is_same<T, c128>::value ? 16 : 32;
+template <typename T, int NB>
+static int blocked_grad_nb(const T* A, int64_t n, void* ws, double* log_out, T* phase_out, T* G, hipStream_t s) {
     if (n % 2 != 0 || (n + BC_T - 1) / BC_T > BC_MAXG) return (int)hipErrorInvalidValue;
     const GJLayout<T> L = gj_layout<T>(n, NB);
     char* base = (char*)ws;
@@ -2645,6 +2652,18 @@ int blocked_grad(const T* A, int64_t n, void* ws, double* log_out, T* phase_out,
     hipLaunchKernelGGL(gj_invperm<T>, dim3(64), dim3(256), 0, s, gp[gc], ig, n, a.st);
     hipLaunchKernelGGL(gj_out<T>, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, s, W[cur], ig, n, a.st, G);
     return (int)hipGetLastError();
+}
+
+template <typename T>
+int blocked_grad(const T* A, int64_t n, void* ws, double* log_out, T* phase_out, T* G, hipStream_t s) {
+    if constexpr (std::is_same<T, c128>::value) {
+        return blocked_grad_nb<T, 16>(A, n, ws, log_out, phase_out, G, s);
+    } else if constexpr (std::is_same<T, double>::value) {
+        if (grad_panel_width(DT_F64) == 32) return blocked_grad_nb<T, 32>(A, n, ws, log_out, phase_out, G, s);
+        return blocked_grad_nb<T, 16>(A, n, ws, log_out, phase_out, G, s);
+    } else {
+        return blocked_grad_nb<T, 32>(A, n, ws, log_out, phase_out, G, s);
+    }
 }
 
 #define PFH_INST(T) \

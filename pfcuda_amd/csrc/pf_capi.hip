@@ -53,6 +53,7 @@ bool tune_value_ok(int key, int v) {
         case PFH_TUNE_BLOCKED_NB: return v == 16 || v == 24 || v == 32;
         case PFH_TUNE_BLOCKED_NB_C128: return v == 8 || v == 16;
         case PFH_TUNE_SWITCH_N: return v >= 64 && v <= 256 && v % 32 == 0;
+        case PFH_TUNE_GRAD_NB: return v == 16 || v == 32;
     }
     return false;
 }

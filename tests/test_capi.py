@@ -107,7 +107,8 @@ def test_set_get_tuning(lib):
         assert lib.pfh_set_tuning(0, 2) == 0
         assert lib.pfh_set_tuning(9, 0) == 0 and lib.pfh_set_tuning(9, 1 << 22) == 0   # panel spin limit
         assert lib.pfh_set_tuning(9, (1 << 22) + 1) == -1 and lib.pfh_set_tuning(9, -2) == -1
-        assert lib.pfh_set_tuning(10, 1) == 0 and lib.pfh_set_tuning(10, 2) == -1   # panel exchange form
+        assert lib.pfh_set_tuning(10, 2) == 0 and lib.pfh_set_tuning(10, 3) == -1   # panel exchange form
+        assert lib.pfh_set_tuning(22, 32) == 0 and lib.pfh_set_tuning(22, 24) == -1  # gradient pairs per group
         assert lib.pfh_set_tuning(11, 9) == 0 and lib.pfh_set_tuning(11, 4) == -1   # mixed digit count
         nkeys = int(re.search(r"#define PFH_TUNE_NKEYS (\d+)", open(HEADER).read()).group(1))
         assert lib.pfh_set_tuning(nkeys, 0) == -1 and lib.pfh_set_tuning(-1, 0) == -1
