@@ -2116,10 +2116,11 @@ int blocked_slog(const T* A, int64_t n, int nb, bool mixed, void* ws, double* lo
 //                which is exactly the factorization's Schur complement, so
 //                the factorization's panel serves unchanged (perm: logical
 //                -> slot on [k0, n); positions below k0 keep their place)
-//   gj_inv       Ai = A_GG^-1 by the unblocked sweep of the 2np x 2np block
-//                (its pivots are already in place: no search)
-//   gj_rows      S = A_GT and Z = Ai A_GT at every column of T, in logical
-//                order; at the columns of G, S = -I and Z = -Ai
+//   gj_rows      Ai = A_GG^-1 by the unblocked sweep of the 2np x 2np block
+//                (its pivots are already in place: no search), in every
+//                block; then S = A_GT and Z = Ai A_GT at the block's columns
+//                of T, in logical order; at the columns of G, S = -I and
+//                Z = -Ai
 //   gj_trailing  Wn[r][c] = base + sum_h S[h][r] Z[h][c] over every upper
 //                tile, base = W[perm r][perm c] for r, c in T, else 0
 // This is the principal pivot transform of the matrix on G,
@@ -2136,26 +2137,26 @@ __device__ __forceinline__ int gj_src(const int* perm, int x, int k0) {
     return x < k0 ? x : p;
 }
 
-// 1024 threads; thread t holds the entries (t / TPR, t % TPR + TPR i) of the
-// small block in registers.  Per pair step the two pivot rows go through LDS
-// (double-buffered by step parity: one barrier per step), with the pivot's
-// reciprocal, taken once by the thread that holds the pivot.  The pair steps
-// are unrolled, so the pivot row and columns are known at compile time: an
-// entry that cannot lie in a pivot column takes the plain update without the
-// selects of the special cases (27 -> 20 us for a 64 x 64 block; a form
-// with 256 threads and 4 x 4 entries each was slower than either).
-template <typename T, int NB>
-__global__ void __launch_bounds__(1024) gj_inv(const T* __restrict__ W, const int* __restrict__ perm, int64_t n,
-                                               int k0, int np, const BPState<T>* st, T* __restrict__ Ai) {
-    constexpr int M = 2 * NB, TPR = 1024 / M, EPT = M / TPR;
+// A_GG^-1 by the unblocked sweep of the group's 2np x 2np block (its pivots
+// are already in place: no search), over the NT threads of a block: thread
+// t holds the entries (t / TPR, t % TPR + TPR i) in registers, returned in
+// v.  Per pair step the two pivot rows go through LDS (double-buffered by
+// step parity: one barrier per step), with the pivot's reciprocal, taken
+// once by the thread that holds the pivot.  The pair steps are unrolled, so
+// the pivot row and columns are known at compile time: an entry that cannot
+// lie in a pivot column takes the plain update without the selects of the
+// special cases.
+template <typename T, int NB, int NT>
+__device__ __forceinline__ void gj_sweep(const T* __restrict__ W, const int* __restrict__ perm, int64_t n, int k0,
+                                         int np, T (&v)[2 * NB / (NT / (2 * NB))]) {
+    constexpr int M = 2 * NB, TPR = NT / M, EPT = M / TPR;
+    static_assert(NT % M == 0 && M % TPR == 0, "the threads tile the block");
     __shared__ T xr[2][M], yr[2][M], rinv[2];
     __shared__ int pg[M];
-    if (st->dead) return;
     const int m = 2 * np, t = threadIdx.x;
     const int r = t / TPR, cq = t % TPR;
     if (t < M) pg[t] = perm[k0 + min(t, m - 1)];
     __syncthreads();
-    T v[EPT];
 #pragma unroll
     for (int i = 0; i < EPT; ++i) {
         const int c = cq + TPR * i;
@@ -2213,8 +2214,6 @@ __global__ void __launch_bounds__(1024) gj_inv(const T* __restrict__ W, const in
     };
     if (m == M) sweep(std::true_type{});
     else sweep(std::false_type{});
-#pragma unroll
-    for (int i = 0; i < EPT; ++i) Ai[r * M + cq + TPR * i] = v[i];
 }
 
 #ifndef GJ_CB64
@@ -2224,12 +2223,16 @@ __global__ void __launch_bounds__(1024) gj_inv(const T* __restrict__ W, const in
 template <int M>
 constexpr int gj_cb() { return M >= 64 ? GJ_CB64 : 8; }
 
+// Each block takes A_GG^-1 itself (gj_sweep, the same for every block) and
+// then forms S and Z at its CB columns; the reads of A_GT are issued before
+// the sweep, so their latency passes under it.
 template <typename T, int NB>
 __global__ void __launch_bounds__(256) gj_rows(const T* __restrict__ W, const int* __restrict__ perm, int64_t n,
-                                               int k0, int np, const BPState<T>* st, const T* __restrict__ Ai,
-                                               T* __restrict__ S, T* __restrict__ Z, const int* __restrict__ gp_old,
+                                               int k0, int np, const BPState<T>* st, T* __restrict__ S,
+                                               T* __restrict__ Z, const int* __restrict__ gp_old,
                                                int* __restrict__ gp_new) {
     constexpr int M = 2 * NB, CB = gj_cb<M>(), HG = 256 / CB, HPT = M / HG;
+    constexpr int TPR = 256 / M, EPT = M / TPR, SPT = (M * CB + 255) / 256;
     static_assert(256 % CB == 0 && M % HG == 0, "row groups tile the block");
     __shared__ T As[M][M + 1];
     __shared__ T Sc[M][CB + 1];
@@ -2237,15 +2240,29 @@ __global__ void __launch_bounds__(256) gj_rows(const T* __restrict__ W, const in
     const int m = 2 * np, t = threadIdx.x;
     const int x0 = blockIdx.x * CB;
     if (t < CB && x0 + t < n) gp_new[x0 + t] = gp_old[gj_src(perm, x0 + t, k0)];
-    for (int e = t; e < M * M; e += 256) As[e / M][e % M] = Ai[e];
-    for (int e = t; e < M * CB; e += 256) {
+    T scv[SPT];
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        const int e = t + 256 * q;
         const int h = e / CB, cx = e % CB, x = x0 + cx;
         T v = zero<T>();
-        if (h < m && x < n) {
+        if (e < M * CB && h < m && x < n) {
             if (gj_in(x, k0, m)) v = (x - k0 == h) ? -one<T>() : zero<T>();
             else v = w_at(W, n, perm[k0 + h], gj_src(perm, x, k0));
         }
-        Sc[h][cx] = v;
+        scv[q] = v;
+    }
+    {
+        T v[EPT];
+        gj_sweep<T, NB, 256>(W, perm, n, k0, np, v);
+        const int r = t / TPR, cq = t % TPR;
+#pragma unroll
+        for (int i = 0; i < EPT; ++i) As[r][cq + TPR * i] = v[i];
+    }
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) {
+        const int e = t + 256 * q;
+        if (e < M * CB) Sc[e / CB][e % CB] = scv[q];
     }
     __syncthreads();
     const int cx = t % CB, hg = t / CB, x = x0 + cx;
@@ -2545,7 +2562,7 @@ __global__ void __launch_bounds__(256) gj_out(const T* __restrict__ W, const int
 template <typename T>
 struct GJLayout {
     BPLayout<T> bp;
-    size_t s, z, ai, gp0, gp1, ig, total;
+    size_t s, z, gp0, gp1, ig, total;
 };
 
 template <typename T>
@@ -2555,7 +2572,6 @@ static GJLayout<T> gj_layout(int64_t n, int nb) {
     size_t o = L.bp.total;
     L.s = o; o = al(o + sizeof(T) * (size_t)2 * nb * n);
     L.z = o; o = al(o + sizeof(T) * (size_t)2 * nb * n);
-    L.ai = o; o = al(o + sizeof(T) * (size_t)4 * nb * nb);
     L.gp0 = o; o = al(o + sizeof(int) * (size_t)n);
     L.gp1 = o; o = al(o + sizeof(int) * (size_t)n);
     L.ig = o; o = al(o + sizeof(int) * (size_t)n);
@@ -2606,7 +2622,6 @@ static int blocked_grad_nb(const T* A, int64_t n, void* ws, double* log_out, T* 
     co.cand = (T*)(base + L.bp.cand);
     T* S = (T*)(base + L.s);
     T* Z = (T*)(base + L.z);
-    T* Ai = (T*)(base + L.ai);
     int* gp[2] = {(int*)(base + L.gp0), (int*)(base + L.gp1)};
     int* ig = (int*)(base + L.ig);
     hipError_t e = hipMemcpyAsync(W[0], A, sizeof(T) * n * n, hipMemcpyDeviceToDevice, s);
@@ -2623,9 +2638,8 @@ static int blocked_grad_nb(const T* A, int64_t n, void* ws, double* log_out, T* 
         a.W = W[cur];
         e = launch_panel<T, NB>(a, co, (int)k0, np, epoch, wepoch, s);
         if (e != hipSuccess) return (int)e;
-        hipLaunchKernelGGL((gj_inv<T, NB>), dim3(1), dim3(1024), 0, s, W[cur], a.perm, n, (int)k0, np, a.st, Ai);
         hipLaunchKernelGGL((gj_rows<T, NB>), dim3((unsigned)((n + gj_cb<2 * NB>() - 1) / gj_cb<2 * NB>())), dim3(256), 0, s, W[cur], a.perm, n,
-                           (int)k0, np, a.st, Ai, S, Z, gp[gc], gp[gc ^ 1]);
+                           (int)k0, np, a.st, S, Z, gp[gc], gp[gc ^ 1]);
         if constexpr (std::is_same<T, double>::value || std::is_same<T, c128>::value) {
             if (trail_staged(PFH_TUNE_TRAIL_SWEEP, true)) {
                 hipLaunchKernelGGL((gj_trailing_staged<T, 2 * NB>), dim3(tiles), dim3(256), 0, s, W[cur], W[cur ^ 1],
