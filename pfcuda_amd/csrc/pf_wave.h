@@ -23,10 +23,6 @@ __device__ __forceinline__ double dpp_f64(double v) {
     const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(b >> 32), (int)(unsigned)(b >> 32), CTRL, ROW_MASK, 0xF, false);
     return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ int dpp_i32(int v) {
-    return __builtin_amdgcn_update_dpp(v, v, CTRL, ROW_MASK, 0xF, false);
-}
 
 // wave maximum (no NaN inputs), in every lane
 __device__ __forceinline__ double wave_max(double m) {
@@ -38,14 +34,19 @@ __device__ __forceinline__ double wave_max(double m) {
     m = fmax(m, dpp_f64<0x143, 0xC>(m));
     return readlane_f64(m, 63);
 }
-// wave minimum, in every lane
+// wave minimum, in every lane; the DPP moves leave INT_MAX (the minimum's
+// identity) in the lanes they do not write, so each folds into the minimum
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_min_i32(int v) {
+    return __builtin_amdgcn_update_dpp(0x7fffffff, v, CTRL, ROW_MASK, 0xF, false);
+}
 __device__ __forceinline__ int wave_min(int k) {
-    k = min(k, dpp_i32<0xB1, 0xF>(k));
-    k = min(k, dpp_i32<0x4E, 0xF>(k));
-    k = min(k, dpp_i32<0x141, 0xF>(k));
-    k = min(k, dpp_i32<0x140, 0xF>(k));
-    k = min(k, dpp_i32<0x142, 0xA>(k));
-    k = min(k, dpp_i32<0x143, 0xC>(k));
+    k = min(k, dpp_min_i32<0xB1, 0xF>(k));
+    k = min(k, dpp_min_i32<0x4E, 0xF>(k));
+    k = min(k, dpp_min_i32<0x141, 0xF>(k));
+    k = min(k, dpp_min_i32<0x140, 0xF>(k));
+    k = min(k, dpp_min_i32<0x142, 0xA>(k));
+    k = min(k, dpp_min_i32<0x143, 0xC>(k));
     return __builtin_amdgcn_readlane(k, 63);
 }
 
@@ -59,11 +60,20 @@ __device__ __forceinline__ double dpp_all_f64(double v) {
     return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
+// DPP move with 0 (the identity of an unsigned maximum) in the lanes the
+// pattern does not write, so that the move folds into the maximum
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned dpp_u32(unsigned v) {
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xF, false);
+}
 // Wave argmax: the largest `best` (inputs >= -1, no NaN) and, among the
 // lanes holding it, the smallest key; (-1, 0x7fffffff) when every lane has
 // -1.  When one lane alone holds the maximum (the usual case) its key is read
 // directly; ties fall back to a min-reduction of the keys, so the result is
-// exactly that of wave_max followed by wave_min.
+// exactly that of wave_max followed by wave_min.  (A form on the 32-bit
+// words of the maximum, one DPP maximum per step, takes fewer instructions
+// but a longer chain of dependent ones: the panel's pair steps, which wait
+// on it, got slower.)
 __device__ __forceinline__ void wave_argmax(double best, int key, double& out_best, int& out_key) {
     double m = best;
     m = fmax(m, dpp_all_f64<0xB1>(m));

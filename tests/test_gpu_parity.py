@@ -39,6 +39,53 @@ def test_c1_batch_1024x32_f64(pf, cuda):
     np.testing.assert_allclose(got, want, rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_two_matrices_per_wave_batches(pf, cuda, dtype):
+    # Real batches of 8192 and more n <= 32 matrices run two per wave: an odd
+    # count (the last wave holds one matrix), ragged sizes with odd and zero
+    # n, integer entries (many exact ties in the pivot search) and a NaN row.
+    # Product form: bit for bit the one-per-wave kernel (the batch in chunks
+    # below the switch) and rel 1e-12 against the oracle (f64); slog of the
+    # ragged batch against the oracle.
+    rng = np.random.default_rng(11)
+    cnt = 8193
+    A = oracle.skew_gaussian(rng, 32, batch=cnt)
+    B = rng.integers(-2, 3, size=(600, 32, 32)).astype(np.float64)
+    A[:600] = np.triu(B, 1) - np.triu(B, 1).transpose(0, 2, 1)
+    A[700, 3, 5:] = np.nan
+    A[700, 5:, 3] = np.nan
+    At = torch.from_numpy(A).to(dtype).to(cuda)
+    got = pf.pfaffian(At)
+    parts = torch.cat([pf.pfaffian(At[i:i + 4096]) for i in range(0, cnt, 4096)])
+    assert torch.equal(torch.nan_to_num(got, nan=7.0), torch.nan_to_num(parts, nan=7.0))
+    got = got.cpu().numpy()
+    assert math.isnan(got[700])
+    lo = dtype == torch.float32
+    for i in list(range(0, 600, 7)) + list(range(600, cnt, 97)) + [cnt - 1]:
+        if i == 700:
+            continue
+        a = A[i].astype(np.float32).astype(np.float64) if lo else A[i]
+        want = oracle.pf_f64(a)[0]
+        tol = (1e-3 * abs(want) + 1e-5) if lo else 1e-12 * max(abs(want), 1.0)
+        assert abs(got[i] - want) <= tol, (i, got[i], want)
+
+    sizes = [int(x) for x in rng.integers(0, 33, size=cnt)]
+    sizes[:6] = [0, 1, 2, 31, 32, 3]
+    mats = [oracle.skew_gaussian(rng, n) if n >= 2 else np.zeros((n, n)) for n in sizes]
+    data = np.concatenate([m.reshape(-1) for m in mats]) if sizes else np.zeros(0)
+    offs = np.cumsum([0] + [n * n for n in sizes[:-1]])
+    gl, gp = pf.slog_pfaffian_ragged(torch.from_numpy(data).to(dtype).to(cuda), offs, sizes)
+    gl, gp = gl.cpu().numpy(), gp.cpu().numpy()
+    src = data.astype(np.float32).astype(np.float64) if lo else data
+    wl, wp = oracle.slog_ragged(src, offs, sizes, threads=8)
+    for i, n in enumerate(sizes):
+        if math.isinf(wl[i]):
+            assert math.isinf(gl[i]) and gl[i] < 0 and gp[i] == 0, (i, n)
+        else:
+            assert abs(gl[i] - wl[i]) < (1e-3 if lo else 1e-9), (i, n, gl[i], wl[i])
+            assert gp[i] == wp[i], (i, n, gp[i], wp[i])
+
+
 @pytest.mark.parametrize("dtype", [torch.float64, torch.complex128, torch.float32, torch.complex64])
 def test_pfaffian_all_sizes(pf, cuda, dtype):
     rng = np.random.default_rng(1)
