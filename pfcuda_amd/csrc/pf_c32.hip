@@ -276,6 +276,7 @@ pf_c32_kernel(BatchDesc d) {
 // bookkeeping) serves two matrices and no register moves, for the price of
 // updating the eliminated rows too.  Every live entry takes the arithmetic
 // of pf_c32_kernel, in the same order, so both give the same bits.
+
 // 32 register rows as two objects of 16, rows 0..15 and 16..31 (a vector of
 // 32 doubles is past what a register index reaches, and one struct holding
 // both vectors went to scratch memory).  Row j at a wave-uniform j: a read
